@@ -1,0 +1,197 @@
+// bfn_internal.hpp -- device-side records of the MI355X engine.
+//
+// Layout in HBM (all tile-major; vertex ids are the reference's reordered ids,
+// so every tile owns one contiguous vertex range, reorder.cpp:266-273):
+//
+//   graph (immutable, shared by every query)
+//     reference image : owner_tiles, CSR/CSC offsets, endpoints, weights,
+//                       CSC edge ids, runs            (DeviceGraphLayout32)
+//     tile directory  : tile_vertex_offsets[T+1]
+//                       in_nbr_offsets[T+1] / in_nbr_tiles[]   (source tiles of
+//                       each tile's incoming edges, the tile itself first)
+//                       out_nbr_offsets[T+1] / out_nbr_tiles[] (same, outgoing)
+//                       in_nbr_stage[]  LDS float offset of each in-neighbour slab
+//     packed edges    : csc_pack[E] = k << 24 | src_local << 12 | dst_local
+//                       (k indexes the destination tile's in-neighbour list)
+//                       csr_pack[E] = k << 24 | src_local << 12 | dst_local
+//                       (k indexes the source tile's out-neighbour list)
+//
+//   batch (one workspace; any number of queries)
+//     slot s = (query q, selected tile T); slots of q are q's selected tiles in
+//     sorted order starting at tile_offsets[q].
+//     slot_dist[s]   float offset of the slot's two npt-long copies in the pool:
+//                    Jacobi uses them as the two distance columns (read r&1,
+//                    write (r+1)&1); push engines keep distances in copy 0 and
+//                    the last-pushed snapshot in copy 1.
+//     slot_in[]/slot_out[]  slot of each in/out-neighbour tile for q, or NONE
+//     stamp[s]       1 + last round s was queued for (dedups the work lists)
+//     lists[3][S]    round-robin work lists; list r%3 is consumed in round r,
+//                    list (r+1)%3 is filled, list (r+2)%3 is reset.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace bfn {
+
+constexpr std::uint32_t kNone = 0xFFFFFFFFu;
+constexpr std::uint32_t kInfBits = 0x7F800000u;  // +inf; non-negative floats order as u32
+constexpr std::uint32_t kMaxTileVertices = 4096;  // 12-bit local ids
+constexpr std::uint32_t kMaxNeighborTiles = 255;  // 8-bit neighbour index
+constexpr std::uint32_t kPackLocalMask = 0xFFFu;
+
+enum Engine : std::uint32_t { kJacobiPull = 0, kDensePush = 1, kFrontierPush = 2 };
+
+enum StopReason : std::uint32_t {
+  kStopNone = 0,
+  kStopConverged = 1,
+  kStopMaximumRounds = 2,
+  kStopQueueOverflow = 3,
+  kStopInvalidController = 4,
+  kStopDeviceFailure = 5,
+};
+
+struct DevGraph {
+  std::uint32_t vertex_count;
+  std::uint32_t edge_count;
+  std::uint32_t tile_count;
+  std::uint32_t max_tile_vertices;
+  std::uint32_t max_stage_floats;
+  std::uint32_t max_in_edges;
+  std::uint32_t max_out_edges;
+  std::uint32_t pad;
+  const std::uint32_t* owner_tiles;
+  const std::uint32_t* tile_vertex_offsets;
+  const std::uint32_t* row_offsets;
+  const std::uint32_t* destinations;
+  const float* weights;
+  const std::uint32_t* column_offsets;
+  const std::uint32_t* sources;
+  const float* in_weights;
+  const std::uint32_t* in_edge_ids;
+  const std::uint32_t* in_nbr_offsets;
+  const std::uint32_t* in_nbr_tiles;
+  const std::uint32_t* in_nbr_stage;
+  const std::uint32_t* out_nbr_offsets;
+  const std::uint32_t* out_nbr_tiles;
+  const std::uint32_t* csc_pack;
+  const std::uint32_t* csr_pack;
+};
+
+// Control block; lives in device memory, one per workspace.
+struct DevControl {
+  std::uint32_t list_count[3];
+  std::uint32_t claim[3];
+  std::uint32_t round;            // next round to execute
+  std::uint32_t done;
+  std::uint32_t stop_reason;
+  std::uint32_t error_bits;
+  std::uint32_t barrier_count;
+  std::uint32_t barrier_generation;
+  std::uint32_t rounds_executed;
+  std::uint32_t pad[3];
+  unsigned long long items_processed;
+  unsigned long long edges_examined;
+  unsigned long long atomic_attempts;
+  unsigned long long atomic_successes;
+  unsigned long long max_list;
+};
+
+struct DevBatch {
+  std::uint32_t query_count;
+  std::uint32_t slot_count;
+  std::uint32_t maximum_rounds;   // 0 = unlimited
+  std::uint32_t tile_sweeps;      // Jacobi: 1 point, 0 closure
+  std::uint32_t items_per_claim;
+  std::uint32_t instrumentation;
+  const std::uint32_t* tile_offsets;    // [Q+1] slots of q
+  const std::uint32_t* tiles;           // [S]   tile of slot
+  const std::uint32_t* source_offsets;  // [Q+1]
+  const std::uint32_t* sources;
+  const std::uint32_t* target_offsets;  // [Q+1]
+  const std::uint32_t* targets;
+  const std::uint32_t* engine;          // [Q]
+  const std::uint32_t* slot_query;      // [S]
+  const unsigned long long* slot_dist;  // [S]
+  const std::uint32_t* slot_in_base;    // [S]
+  const std::uint32_t* slot_out_base;   // [S]
+  std::uint32_t* slot_in;
+  std::uint32_t* slot_out;
+  std::uint32_t* stamp;                 // [S]
+  std::uint32_t* lists;                 // [3*S]
+  std::uint32_t* dist;                  // float bits pool
+  std::uint32_t* last_change;           // [Q] 1 + last round with a decrease
+  DevControl* control;
+};
+
+// Per-query outcome: rounds, convergence, all-targets reach.
+struct QueryStatusDev {
+  unsigned long long rounds;
+  unsigned long long relaxations;
+  std::uint32_t converged;
+  std::uint32_t reached;
+  std::uint32_t miss;
+  std::uint32_t stop_reason;
+  std::uint32_t error_bits;
+  std::uint32_t engine;
+};
+
+// Arguments of the compact-path passes.
+struct PathArgs {
+  DevGraph g;
+  DevBatch b;
+  std::uint32_t jacobi_copy;            // column holding final Jacobi labels
+  std::uint32_t pass;                   // 1: summaries + lengths, 2: emit arenas
+  std::uint32_t target_total;
+  std::uint32_t pad;
+  const std::uint32_t* query_clean;     // [Q] converged without error
+  const unsigned long long* frame_base; // [Q] frame scratch offset
+  std::uint32_t* frame_vertex;
+  std::uint32_t* frame_last;
+  std::uint32_t* frame_via;
+  std::uint32_t* onpath;                // pool-sized stamps
+  std::uint32_t* summaries;             // [targets][7]
+  unsigned long long* lengths;          // [targets] vertex count of each path
+  const unsigned long long* offsets;    // [targets+1] (pass 2)
+  std::uint32_t* arena_vertices;
+  std::uint32_t* arena_labels;
+  std::uint32_t* arena_edges;
+};
+
+// ---- memory helpers -------------------------------------------------------
+// Loads of values other workgroups may be lowering concurrently go through
+// agent-scope atomic loads (coherent across XCDs); monotone relaxation makes a
+// stale value safe, freshness only speeds convergence.
+__device__ __forceinline__ std::uint32_t load_agent(const std::uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_agent(std::uint32_t* p, std::uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ std::uint32_t atomic_min_agent(std::uint32_t* p, std::uint32_t v) {
+  return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ std::uint32_t atomic_max_agent(std::uint32_t* p, std::uint32_t v) {
+  return __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ std::uint32_t atomic_add_agent(std::uint32_t* p, std::uint32_t v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ float as_float(std::uint32_t b) { return __uint_as_float(b); }
+__device__ __forceinline__ std::uint32_t as_bits(float f) { return __float_as_uint(f); }
+
+// Binary search of tile in the sorted selected-tile list of a query.
+__device__ __forceinline__ std::uint32_t find_slot(const std::uint32_t* tiles,
+                                                   std::uint32_t begin, std::uint32_t end,
+                                                   std::uint32_t tile) {
+  std::uint32_t lo = begin, hi = end;
+  while (lo < hi) {
+    std::uint32_t mid = (lo + hi) >> 1;
+    if (tiles[mid] < tile) lo = mid + 1; else hi = mid;
+  }
+  return (lo < end && tiles[lo] == tile) ? lo : kNone;
+}
+
+}  // namespace bfn

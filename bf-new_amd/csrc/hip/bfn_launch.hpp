@@ -1,0 +1,51 @@
+// bfn_launch.hpp -- host-callable launch wrappers of the engine kernels.
+#pragma once
+
+#include "bfn_internal.hpp"
+
+namespace bfn {
+
+struct PathArgs;
+
+hipError_t relax_launch(std::uint32_t block, bool persistent, bool cooperative,
+                        std::uint32_t grid, std::size_t lds_bytes, std::uint32_t max_v,
+                        std::uint32_t work_words, const DevGraph& g, const DevBatch& b,
+                        hipStream_t stream);
+hipError_t relax_occupancy(std::uint32_t block, bool persistent, std::size_t lds_bytes,
+                           int* blocks_per_cu);
+hipError_t prep_launch(const DevGraph& g, const DevBatch& b, std::uint64_t pool_words,
+                       hipStream_t stream);
+hipError_t derive_tile_image(std::uint32_t tile_count, const std::uint32_t* tile_vertex_offsets,
+                             const std::uint32_t* owner_tiles, const std::uint32_t* row_offsets,
+                             const std::uint32_t* destinations, const std::uint32_t* row_runs,
+                             const std::uint32_t* run_dst_tiles, const float* weights,
+                             const std::uint32_t* column_offsets, const std::uint32_t* sources,
+                             const std::uint32_t* col_runs, const std::uint32_t* run_src_tiles,
+                             const float* in_weights, std::uint32_t* counts_scratch,
+                             std::uint32_t* in_offsets, std::uint32_t* out_offsets,
+                             std::uint32_t** in_tiles, std::uint32_t** in_stage,
+                             std::uint32_t** out_tiles, std::uint32_t* csc_pack,
+                             std::uint32_t* csr_pack, std::uint32_t* stats,
+                             std::uint32_t* host_in_total, std::uint32_t* host_out_total,
+                             hipStream_t stream);
+hipError_t paths_launch(const PathArgs& a, std::uint32_t query_count, hipStream_t stream);
+hipError_t scan_u64_launch(const unsigned long long* in, unsigned long long* out, std::uint32_t n,
+                           hipStream_t stream);
+hipError_t predecessors_launch(const DevGraph& g, const DevBatch& b, std::uint32_t q,
+                               std::uint32_t slots, std::uint32_t copy, std::uint32_t* out,
+                               hipStream_t stream);
+hipError_t scatter_launch(const DevGraph& g, const DevBatch& b, std::uint32_t q,
+                          std::uint32_t slots, std::uint32_t copy, std::uint32_t* out,
+                          hipStream_t stream);
+hipError_t finalize_launch(const DevGraph& g, const DevBatch& b,
+                           const unsigned long long* relaxations, void* out, hipStream_t stream);
+hipError_t relaxation_count_launch(const DevGraph& g, const DevBatch& b,
+                                   unsigned long long* out, hipStream_t stream);
+hipError_t occupancy_launch(const std::uint32_t* arena, unsigned long long n, int* occ,
+                            hipStream_t stream);
+hipError_t node_cost_launch(const DevGraph& g, const float* base, const int* occ, float* csr_w,
+                            float* csc_w, hipStream_t stream);
+hipError_t fill_launch(std::uint32_t* p, unsigned long long n, std::uint32_t value,
+                       hipStream_t stream);
+
+}  // namespace bfn

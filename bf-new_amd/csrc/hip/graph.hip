@@ -1,0 +1,230 @@
+// graph.hip -- derives the tile-major engine image from the reference layout.
+//
+// Input: DeviceGraphLayout32 (device_layout.hpp:72-93) already in HBM, with
+// the tile runs of device_layout.cpp:186-251 and tile vertex offsets.
+// Output, per tile T:
+//   in_nbr_tiles  -- T, then the other source tiles of T's CSC runs, ascending
+//   out_nbr_tiles -- T, then the other destination tiles of T's CSR runs
+//   in_nbr_stage  -- LDS float offset of each in-neighbour slab (prefix of
+//                    tile sizes in list order, so T's own slab is at 0)
+//   csc_pack / csr_pack -- one u32 per edge: neighbour index << 24 |
+//                    source-local id << 12 | destination-local id
+// One workgroup per tile; the distinct tiles of the tile's runs are collected
+// in an LDS hash set, ranked in LDS, and the packs are written per vertex.
+#include "bfn_internal.hpp"
+
+namespace bfn {
+
+namespace {
+
+constexpr std::uint32_t kHashSize = 1024;
+
+struct DeriveArgs {
+  std::uint32_t tile_count;
+  const std::uint32_t* tile_vertex_offsets;
+  const std::uint32_t* owner_tiles;
+  // orientation: CSC (in) or CSR (out)
+  const std::uint32_t* offsets;        // [V+1]
+  const std::uint32_t* endpoints;      // sources (CSC) or destinations (CSR)
+  const std::uint32_t* run_offsets;    // [V+1] runs per vertex
+  const std::uint32_t* run_tiles;      // remote tile per run
+  std::uint32_t* counts;               // [T] distinct neighbour tiles (incl. T)
+  const std::uint32_t* list_offsets;   // [T+1] (fill/pack passes)
+  std::uint32_t* list_tiles;
+  std::uint32_t* list_stage;           // nullable (CSR has no staging)
+  std::uint32_t* pack;                 // [E]
+  std::uint32_t* stats;                // [0] error, [1] max stage floats, [2] max CSC edges
+                                       // per tile, [3] max neighbours, [4] zero weight,
+                                       // [5] max CSR edges per tile
+  const float* weights;
+  bool csc;
+};
+
+__device__ void insert(std::uint32_t* set, std::uint32_t key, std::uint32_t* overflow) {
+  std::uint32_t h = (key * 2654435761u) >> 22;
+  for (std::uint32_t probe = 0; probe < kHashSize; ++probe) {
+    const std::uint32_t slot = (h + probe) & (kHashSize - 1);
+    const std::uint32_t prev = atomicCAS(&set[slot], kNone, key);
+    if (prev == kNone || prev == key) return;
+  }
+  *overflow = 1;
+}
+
+// Collects the distinct tiles into `set`; returns the compacted, ranked list
+// in `list` (T first) and its length.
+__device__ std::uint32_t collect(const DeriveArgs& a, std::uint32_t T, std::uint32_t* set,
+                                 std::uint32_t* list, std::uint32_t* scratch) {
+  const std::uint32_t tid = threadIdx.x;
+  for (std::uint32_t i = tid; i < kHashSize; i += blockDim.x) set[i] = kNone;
+  if (tid == 0) {
+    scratch[0] = 0;  // overflow
+    scratch[1] = 0;  // count
+  }
+  __syncthreads();
+  if (tid == 0) insert(set, T, &scratch[0]);
+  const std::uint32_t v0 = a.tile_vertex_offsets[T], v1 = a.tile_vertex_offsets[T + 1];
+  const std::uint32_t r0 = a.run_offsets[v0], r1 = a.run_offsets[v1];
+  for (std::uint32_t r = r0 + tid; r < r1; r += blockDim.x) insert(set, a.run_tiles[r], &scratch[0]);
+  __syncthreads();
+  // compact
+  for (std::uint32_t i = tid; i < kHashSize; i += blockDim.x) {
+    if (set[i] != kNone) {
+      const std::uint32_t pos = atomicAdd(&scratch[1], 1u);
+      scratch[2 + pos] = set[i];
+    }
+  }
+  __syncthreads();
+  const std::uint32_t n = scratch[1];
+  // rank: T first, the rest ascending
+  for (std::uint32_t i = tid; i < n; i += blockDim.x) {
+    const std::uint32_t key = scratch[2 + i];
+    std::uint32_t rank = 0;
+    if (key != T) {
+      rank = 1;
+      for (std::uint32_t j = 0; j < n; ++j) {
+        const std::uint32_t other = scratch[2 + j];
+        if (other != T && other < key) ++rank;
+      }
+    }
+    list[rank] = key;
+  }
+  __syncthreads();
+  if (tid == 0 && (scratch[0] || n > kMaxNeighborTiles)) atomicOr(&a.stats[0], 1u);
+  return n;
+}
+
+__global__ void count_kernel(DeriveArgs a) {
+  __shared__ std::uint32_t set[kHashSize];
+  __shared__ std::uint32_t list[kHashSize];
+  __shared__ std::uint32_t scratch[kHashSize + 2];
+  const std::uint32_t T = blockIdx.x;
+  const std::uint32_t n = collect(a, T, set, list, scratch);
+  if (threadIdx.x == 0) {
+    a.counts[T] = n;
+    atomicMax(&a.stats[3], n);
+    const std::uint32_t v0 = a.tile_vertex_offsets[T], v1 = a.tile_vertex_offsets[T + 1];
+    atomicMax(&a.stats[a.csc ? 2 : 5], a.offsets[v1] - a.offsets[v0]);
+  }
+}
+
+__global__ void fill_pack_kernel(DeriveArgs a) {
+  __shared__ std::uint32_t set[kHashSize];
+  __shared__ std::uint32_t list[kHashSize];
+  __shared__ std::uint32_t scratch[kHashSize + 2];
+  const std::uint32_t T = blockIdx.x;
+  const std::uint32_t n = collect(a, T, set, list, scratch);
+  const std::uint32_t tid = threadIdx.x;
+  const std::uint32_t base = a.list_offsets[T];
+  if (n > kMaxNeighborTiles || a.list_offsets[T + 1] - base != n) return;
+  for (std::uint32_t i = tid; i < n; i += blockDim.x) a.list_tiles[base + i] = list[i];
+  if (a.list_stage && tid == 0) {
+    std::uint32_t off = 0;
+    for (std::uint32_t i = 0; i < n; ++i) {
+      a.list_stage[base + i] = off;
+      off += a.tile_vertex_offsets[list[i] + 1] - a.tile_vertex_offsets[list[i]];
+    }
+    atomicMax(&a.stats[1], off);
+  }
+  const std::uint32_t v0 = a.tile_vertex_offsets[T], v1 = a.tile_vertex_offsets[T + 1];
+  for (std::uint32_t v = v0 + tid; v < v1; v += blockDim.x) {
+    for (std::uint32_t e = a.offsets[v]; e < a.offsets[v + 1]; ++e) {
+      const std::uint32_t other = a.endpoints[e];
+      const std::uint32_t ot = a.owner_tiles[other];
+      std::uint32_t k = 0;
+      if (ot != T) {
+        std::uint32_t lo = 1, hi = n;
+        while (lo < hi) {
+          const std::uint32_t mid = (lo + hi) >> 1;
+          if (list[mid] < ot) lo = mid + 1; else hi = mid;
+        }
+        k = lo;
+      }
+      const std::uint32_t other_local = other - a.tile_vertex_offsets[ot];
+      const std::uint32_t self_local = v - v0;
+      // CSC: the edge runs other -> v; CSR: v -> other.
+      const std::uint32_t src_local = a.csc ? other_local : self_local;
+      const std::uint32_t dst_local = a.csc ? self_local : other_local;
+      a.pack[e] = (k << 24) | (src_local << 12) | dst_local;
+      if (a.weights[e] == 0.0f) a.stats[4] = 1;
+    }
+  }
+}
+
+__global__ void exclusive_scan_kernel(const std::uint32_t* in, std::uint32_t* out,
+                                      std::uint32_t n) {
+  // Single workgroup: chunked Hillis-Steele in LDS, carry across chunks.
+  __shared__ std::uint32_t buf[1024];
+  __shared__ std::uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (std::uint32_t base = 0; base < n; base += 1024) {
+    const std::uint32_t i = base + threadIdx.x;
+    std::uint32_t x = i < n ? in[i] : 0;
+    buf[threadIdx.x] = x;
+    __syncthreads();
+    for (std::uint32_t off = 1; off < 1024; off <<= 1) {
+      std::uint32_t y = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+      __syncthreads();
+      buf[threadIdx.x] += y;
+      __syncthreads();
+    }
+    if (i < n) out[i] = carry + buf[threadIdx.x] - x;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += buf[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[n] = carry;
+}
+
+}  // namespace
+
+// Runs both orientations.  stats[0] != 0 means a tile has more than 255
+// neighbour tiles (not representable in the 8-bit neighbour index).
+hipError_t derive_tile_image(std::uint32_t tile_count, const std::uint32_t* tile_vertex_offsets,
+                             const std::uint32_t* owner_tiles, const std::uint32_t* row_offsets,
+                             const std::uint32_t* destinations, const std::uint32_t* row_runs,
+                             const std::uint32_t* run_dst_tiles, const float* weights,
+                             const std::uint32_t* column_offsets, const std::uint32_t* sources,
+                             const std::uint32_t* col_runs, const std::uint32_t* run_src_tiles,
+                             const float* in_weights, std::uint32_t* counts_scratch,
+                             std::uint32_t* in_offsets, std::uint32_t* out_offsets,
+                             std::uint32_t** in_tiles, std::uint32_t** in_stage,
+                             std::uint32_t** out_tiles, std::uint32_t* csc_pack,
+                             std::uint32_t* csr_pack, std::uint32_t* stats,
+                             std::uint32_t* host_in_total, std::uint32_t* host_out_total,
+                             hipStream_t stream) {
+  DeriveArgs in{tile_count, tile_vertex_offsets, owner_tiles, column_offsets, sources, col_runs,
+                run_src_tiles, counts_scratch, in_offsets, nullptr, nullptr, csc_pack, stats,
+                in_weights, true};
+  DeriveArgs out{tile_count, tile_vertex_offsets, owner_tiles, row_offsets, destinations, row_runs,
+                 run_dst_tiles, counts_scratch, out_offsets, nullptr, nullptr, csr_pack, stats,
+                 weights, false};
+  hipError_t e;
+  // In-neighbour lists.
+  hipLaunchKernelGGL(count_kernel, dim3(tile_count), dim3(256), 0, stream, in);
+  hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(1024), 0, stream, counts_scratch,
+                     in_offsets, tile_count);
+  if ((e = hipMemcpyAsync(host_in_total, in_offsets + tile_count, sizeof(std::uint32_t),
+                          hipMemcpyDeviceToHost, stream)) != hipSuccess)
+    return e;
+  if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
+  if ((e = hipMalloc(in_tiles, sizeof(std::uint32_t) * (*host_in_total + 1))) != hipSuccess) return e;
+  if ((e = hipMalloc(in_stage, sizeof(std::uint32_t) * (*host_in_total + 1))) != hipSuccess) return e;
+  in.list_tiles = *in_tiles;
+  in.list_stage = *in_stage;
+  hipLaunchKernelGGL(fill_pack_kernel, dim3(tile_count), dim3(256), 0, stream, in);
+  // Out-neighbour lists.
+  hipLaunchKernelGGL(count_kernel, dim3(tile_count), dim3(256), 0, stream, out);
+  hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(1024), 0, stream, counts_scratch,
+                     out_offsets, tile_count);
+  if ((e = hipMemcpyAsync(host_out_total, out_offsets + tile_count, sizeof(std::uint32_t),
+                          hipMemcpyDeviceToHost, stream)) != hipSuccess)
+    return e;
+  if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
+  if ((e = hipMalloc(out_tiles, sizeof(std::uint32_t) * (*host_out_total + 1))) != hipSuccess) return e;
+  out.list_tiles = *out_tiles;
+  hipLaunchKernelGGL(fill_pack_kernel, dim3(tile_count), dim3(256), 0, stream, out);
+  return hipGetLastError();
+}
+
+}  // namespace bfn

@@ -1,0 +1,580 @@
+// relax.hip -- the three relaxation engines as tile visits, and their control.
+//
+// Work unit: a slot s = (query q, selected tile T).  One workgroup visits one
+// slot at a time.  Edges of the tile are read once per visit, coalesced, into
+// registers (thread t holds edges t, t+B, t+2B, ... of the tile's contiguous
+// CSC or CSR block); the distances the visit touches are staged in LDS and
+// relaxed there with ds_min_u32 on the order-preserving u32 image of the fp32
+// distance (all distances are +0, finite positive or +inf, so unsigned order
+// equals float order).
+//
+// Semantics (reference engines):
+//   Jacobi pull   batched_jacobi.hip.cpp:466-694 -- two columns, round r reads
+//                 column r&1 and writes column (r+1)&1:
+//                 d'[v] = min(d[v], min over admitted in-edges fl(d[u] + w)).
+//                 tile_sweeps = 1 is exactly that (same rounds as the
+//                 reference); tile_sweeps = 0 additionally closes the tile's
+//                 internal edges in LDS each round (block Jacobi: the same
+//                 fixed point, fewer rounds).
+//   dense push    batched_dense_chaotic_push.hip.cpp:392-627 -- one column,
+//                 in place, atomicMin(d[dst], fl(d[src] + w)) over every
+//                 admitted out-edge of a dirty tile.
+//   frontier push batched_frontier_push.hip.cpp:710-906 -- as dense, but
+//                 only from vertices whose label dropped below the label last
+//                 pushed (column 1 holds that snapshot).
+// Every schedule reaches the same bits: fl(x + w) is monotone in x, so the
+// relaxation operator has one greatest fixed point.
+//
+// Admission: a neighbour tile that the query did not select is staged as
+// +inf (pull) or has no slot (push), so edges from or to it never relax.
+#include "bfn_internal.hpp"
+
+namespace bfn {
+
+// LDS carve-up (u32 words): [own | orig | work | table(512) | flags(8)], with
+// `work` = staged neighbour slabs (pull) or the pushed snapshot (push), and
+// `table` = per-neighbour LDS offsets (pull) or neighbour slots followed by
+// per-neighbour "pushed into" marks (push).
+struct Lds {
+  std::uint32_t* own;
+  std::uint32_t* orig;
+  std::uint32_t* work;
+  std::uint32_t* table;
+  std::uint32_t* flags;
+};
+
+__device__ __forceinline__ Lds carve(std::uint32_t* base, std::uint32_t max_v,
+                                     std::uint32_t work_words) {
+  Lds l;
+  l.own = base;
+  l.orig = base + max_v;
+  l.work = base + 2 * max_v;
+  l.table = l.work + work_words;
+  l.flags = l.table + 512;
+  return l;
+}
+
+__device__ __forceinline__ void enqueue(const DevBatch& b, std::uint32_t slot,
+                                        std::uint32_t round) {
+  const std::uint32_t tag = round + 1;
+  const std::uint32_t old = atomic_max_agent(&b.stamp[slot], tag);
+  if (old < tag) {
+    const std::uint32_t li = round % 3;
+    const std::uint32_t pos = atomic_add_agent(&b.control->list_count[li], 1u);
+    b.lists[static_cast<std::size_t>(li) * b.slot_count + pos] = slot;
+  }
+}
+
+template <int B, int EPT>
+struct EdgeRegs {
+  std::uint32_t pack[EPT];
+  float w[EPT];
+};
+
+// Loads chunk [base, base + B*EPT) of a tile's edge block into registers.
+template <int B, int EPT>
+__device__ __forceinline__ void load_edges(EdgeRegs<B, EPT>& r, const std::uint32_t* pack,
+                                           const float* w, std::uint32_t base,
+                                           std::uint32_t end) {
+#pragma unroll
+  for (int j = 0; j < EPT; ++j) {
+    const std::uint32_t e = base + j * B + threadIdx.x;
+    if (e < end) {
+      r.pack[j] = pack[e];
+      r.w[j] = w[e];
+    } else {
+      r.pack[j] = kNone;
+      r.w[j] = 0.0f;
+    }
+  }
+}
+
+__device__ __forceinline__ std::uint32_t pk_k(std::uint32_t p) { return p >> 24; }
+__device__ __forceinline__ std::uint32_t pk_src(std::uint32_t p) { return (p >> 12) & kPackLocalMask; }
+__device__ __forceinline__ std::uint32_t pk_dst(std::uint32_t p) { return p & kPackLocalMask; }
+
+// ---------------------------------------------------------------------------
+// Jacobi pull visit.
+// ---------------------------------------------------------------------------
+template <int B, int EPT>
+__device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b, std::uint32_t s,
+                           std::uint32_t r, const Lds& l, EdgeRegs<B, EPT>& regs) {
+  const std::uint32_t tid = threadIdx.x;
+  const std::uint32_t T = b.tiles[s];
+  const std::uint32_t q = b.slot_query[s];
+  const std::uint32_t v0 = g.tile_vertex_offsets[T];
+  const std::uint32_t npt = g.tile_vertex_offsets[T + 1] - v0;
+  const std::uint32_t nb0 = g.in_nbr_offsets[T];
+  const std::uint32_t K = g.in_nbr_offsets[T + 1] - nb0;
+  const std::uint32_t rp = r & 1u;
+  const std::uint32_t in_base = b.slot_in_base[s];
+
+  for (std::uint32_t k = tid; k < K; k += B) l.table[k] = g.in_nbr_stage[nb0 + k];
+  __syncthreads();
+  // Stage the read column of every in-neighbour slab (k = 0 is T itself).
+  for (std::uint32_t k = 0; k < K; ++k) {
+    const std::uint32_t nt = g.in_nbr_tiles[nb0 + k];
+    const std::uint32_t n = g.tile_vertex_offsets[nt + 1] - g.tile_vertex_offsets[nt];
+    const std::uint32_t ns = b.slot_in[in_base + k];
+    std::uint32_t* dst = l.work + l.table[k];
+    if (ns == kNone) {
+      for (std::uint32_t i = tid; i < n; i += B) dst[i] = kInfBits;
+    } else {
+      const std::uint32_t* src = b.dist + b.slot_dist[ns] + static_cast<std::size_t>(rp) * n;
+      for (std::uint32_t i = tid; i < n; i += B) dst[i] = src[i];
+    }
+  }
+  __syncthreads();
+  for (std::uint32_t i = tid; i < npt; i += B) l.own[i] = l.work[i];
+  __syncthreads();
+
+  const std::uint32_t e0 = g.column_offsets[v0];
+  const std::uint32_t e1 = g.column_offsets[v0 + npt];
+  const bool single = (e1 - e0) <= static_cast<std::uint32_t>(B * EPT);
+  // Pass 1: every admitted in-edge reads the staged read column.
+  for (std::uint32_t base = e0; base < e1; base += B * EPT) {
+    load_edges<B, EPT>(regs, g.csc_pack, g.in_weights, base, e1);
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      const std::uint32_t p = regs.pack[j];
+      if (p == kNone) continue;
+      const float cand = as_float(l.work[l.table[pk_k(p)] + pk_src(p)]) + regs.w[j];
+      const std::uint32_t cb = as_bits(cand);
+      if (cb < l.own[pk_dst(p)]) atomicMin(&l.own[pk_dst(p)], cb);
+    }
+  }
+  __syncthreads();
+  if (b.tile_sweeps == 0) {
+    // Close the tile's internal edges in place (k == 0 sources are in T).
+    for (;;) {
+      bool changed = false;
+      for (std::uint32_t base = e0; base < e1; base += B * EPT) {
+        if (!single) load_edges<B, EPT>(regs, g.csc_pack, g.in_weights, base, e1);
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) {
+          const std::uint32_t p = regs.pack[j];
+          if (p == kNone || pk_k(p) != 0) continue;
+          const float cand = as_float(l.own[pk_src(p)]) + regs.w[j];
+          const std::uint32_t cb = as_bits(cand);
+          if (cb < l.own[pk_dst(p)]) {
+            atomicMin(&l.own[pk_dst(p)], cb);
+            changed = true;
+          }
+        }
+      }
+      if (!__syncthreads_or(changed)) break;
+    }
+  }
+
+  std::uint32_t* out = b.dist + b.slot_dist[s] + static_cast<std::size_t>(rp ^ 1u) * npt;
+  bool changed = false;
+  for (std::uint32_t i = tid; i < npt; i += B) {
+    const std::uint32_t v = l.own[i];
+    out[i] = v;
+    changed |= (v != l.work[i]);
+  }
+  if (__syncthreads_or(changed)) {
+    if (tid == 0) atomic_max_agent(&b.last_change[q], r + 1);
+    const std::uint32_t ob0 = g.out_nbr_offsets[T];
+    const std::uint32_t KO = g.out_nbr_offsets[T + 1] - ob0;
+    const std::uint32_t out_base = b.slot_out_base[s];
+    for (std::uint32_t j = tid; j < KO; j += B) {
+      const std::uint32_t ns = b.slot_out[out_base + j];
+      if (ns != kNone) enqueue(b, ns, r + 1);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Dense / frontier push visit.
+// ---------------------------------------------------------------------------
+template <int B, int EPT>
+__device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b, std::uint32_t s,
+                           std::uint32_t r, bool frontier, const Lds& l,
+                           EdgeRegs<B, EPT>& regs, unsigned long long& attempts,
+                           unsigned long long& successes) {
+  const std::uint32_t tid = threadIdx.x;
+  const std::uint32_t T = b.tiles[s];
+  const std::uint32_t q = b.slot_query[s];
+  const std::uint32_t v0 = g.tile_vertex_offsets[T];
+  const std::uint32_t npt = g.tile_vertex_offsets[T + 1] - v0;
+  const std::uint32_t ob0 = g.out_nbr_offsets[T];
+  const std::uint32_t KO = g.out_nbr_offsets[T + 1] - ob0;
+  const std::uint32_t out_base = b.slot_out_base[s];
+  std::uint32_t* column = b.dist + b.slot_dist[s];
+  std::uint32_t* snapshot = column + npt;
+
+  for (std::uint32_t k = tid; k < KO; k += B) {
+    l.table[k] = b.slot_out[out_base + k];
+    l.table[256 + k] = 0;
+  }
+  for (std::uint32_t i = tid; i < npt; i += B) {
+    const std::uint32_t x = load_agent(column + i);
+    l.own[i] = x;
+    l.orig[i] = x;
+    // Dense pushes every finite vertex: an infinite snapshot admits them all.
+    l.work[i] = frontier ? snapshot[i] : kInfBits;
+  }
+  __syncthreads();
+
+  const std::uint32_t e0 = g.row_offsets[v0];
+  const std::uint32_t e1 = g.row_offsets[v0 + npt];
+  const bool single = (e1 - e0) <= static_cast<std::uint32_t>(B * EPT);
+  if (single) load_edges<B, EPT>(regs, g.csr_pack, g.weights, e0, e1);
+  // Internal edges to the tile's local fixed point, from frontier sources.
+  for (;;) {
+    bool changed = false;
+    for (std::uint32_t base = e0; base < e1; base += B * EPT) {
+      if (!single) load_edges<B, EPT>(regs, g.csr_pack, g.weights, base, e1);
+#pragma unroll
+      for (int j = 0; j < EPT; ++j) {
+        const std::uint32_t p = regs.pack[j];
+        if (p == kNone || pk_k(p) != 0) continue;
+        const std::uint32_t sv = l.own[pk_src(p)];
+        if (!(sv < l.work[pk_src(p)])) continue;
+        const std::uint32_t cb = as_bits(as_float(sv) + regs.w[j]);
+        if (cb < l.own[pk_dst(p)]) {
+          atomicMin(&l.own[pk_dst(p)], cb);
+          changed = true;
+        }
+      }
+    }
+    if (!__syncthreads_or(changed)) break;
+  }
+  // Boundary edges: push into neighbour slots with a global atomic min.
+  unsigned long long att = 0, suc = 0;
+  for (std::uint32_t base = e0; base < e1; base += B * EPT) {
+    if (!single) load_edges<B, EPT>(regs, g.csr_pack, g.weights, base, e1);
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      const std::uint32_t p = regs.pack[j];
+      if (p == kNone || pk_k(p) == 0) continue;
+      const std::uint32_t sv = l.own[pk_src(p)];
+      if (!(sv < l.work[pk_src(p)])) continue;
+      const std::uint32_t ns = l.table[pk_k(p)];
+      if (ns == kNone) continue;
+      const std::uint32_t cb = as_bits(as_float(sv) + regs.w[j]);
+      std::uint32_t* target = b.dist + b.slot_dist[ns] + pk_dst(p);
+      ++att;
+      // Read first: most candidates lose once labels settle.
+      if (cb < load_agent(target)) {
+        const std::uint32_t old = atomic_min_agent(target, cb);
+        if (cb < old) {
+          ++suc;
+          l.table[256 + pk_k(p)] = 1;
+        }
+      }
+    }
+  }
+  attempts += att;
+  successes += suc;
+  bool changed = false;
+  for (std::uint32_t i = tid; i < npt; i += B) {
+    const std::uint32_t v = l.own[i];
+    if (v < l.orig[i]) {
+      atomic_min_agent(column + i, v);
+      changed = true;
+    }
+    if (frontier && v < l.work[i]) snapshot[i] = v;
+  }
+  const bool pushed_any = __syncthreads_or(changed);
+  bool marked = false;
+  for (std::uint32_t k = 1 + tid; k < KO; k += B) {
+    if (l.table[256 + k]) {
+      enqueue(b, l.table[k], r + 1);
+      marked = true;
+    }
+  }
+  if (__syncthreads_or(marked) || pushed_any) {
+    if (tid == 0) atomic_max_agent(&b.last_change[q], r + 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Round processing shared by the persistent and host-polled kernels.
+// ---------------------------------------------------------------------------
+template <int B, int EPT>
+__device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch& b, std::uint32_t r,
+                              std::uint32_t n, const Lds& l) {
+  const std::uint32_t li = r % 3;
+  const std::uint32_t chunk = b.items_per_claim ? b.items_per_claim : 1u;
+  EdgeRegs<B, EPT> regs;
+  unsigned long long attempts = 0, successes = 0, items = 0;
+  for (;;) {
+    if (threadIdx.x == 0) l.flags[0] = atomic_add_agent(&b.control->claim[li], chunk);
+    __syncthreads();
+    const std::uint32_t start = l.flags[0];
+    __syncthreads();
+    if (start >= n) break;
+    const std::uint32_t end = min(start + chunk, n);
+    for (std::uint32_t i = start; i < end; ++i) {
+      const std::uint32_t s = b.lists[static_cast<std::size_t>(li) * b.slot_count + i];
+      const std::uint32_t engine = b.engine[b.slot_query[s]];
+      if (engine == kJacobiPull) {
+        visit_pull<B, EPT>(g, b, s, r, l, regs);
+      } else {
+        visit_push<B, EPT>(g, b, s, r, engine == kFrontierPush, l, regs, attempts, successes);
+      }
+      ++items;
+      __syncthreads();
+    }
+  }
+  if (b.instrumentation) {
+    // One add per thread keeps the counters off the hot path's critical lanes.
+    if (attempts) atomicAdd(&b.control->atomic_attempts, attempts);
+    if (successes) atomicAdd(&b.control->atomic_successes, successes);
+    if (threadIdx.x == 0 && items) atomicAdd(&b.control->items_processed, items);
+  }
+}
+
+__device__ __forceinline__ void grid_barrier(DevControl* c, std::uint32_t blocks) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const std::uint32_t gen = load_agent(&c->barrier_generation);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const std::uint32_t arrived = atomic_add_agent(&c->barrier_count, 1u);
+    if (arrived == blocks - 1) {
+      store_agent(&c->barrier_count, 0u);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      store_agent(&c->barrier_generation, gen + 1);
+    } else {
+      while (load_agent(&c->barrier_generation) == gen) __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+// One cooperative launch runs every round: process list r%3, grid barrier,
+// repeat while the next list is non-empty and the round limit allows.
+template <int B, int EPT>
+__global__ __launch_bounds__(B) void relax_persistent_kernel(DevGraph g, DevBatch b,
+                                                             std::uint32_t max_v,
+                                                             std::uint32_t work_words) {
+  extern __shared__ std::uint32_t lds_words[];
+  const Lds l = carve(lds_words, max_v, work_words);
+  DevControl* c = b.control;
+  std::uint32_t r = load_agent(&c->round);
+  if (load_agent(&c->done)) return;
+  std::uint32_t n = 0;
+  for (;;) {
+    n = load_agent(&c->list_count[r % 3]);
+    if (n == 0 || (b.maximum_rounds && r >= b.maximum_rounds)) break;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      store_agent(&c->list_count[(r + 2) % 3], 0u);
+      store_agent(&c->claim[(r + 2) % 3], 0u);
+      if (b.instrumentation) atomicMax(&c->max_list, static_cast<unsigned long long>(n));
+    }
+    process_round<B, EPT>(g, b, r, n, l);
+    grid_barrier(c, gridDim.x);
+    ++r;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    c->round = r;
+    c->rounds_executed = r;
+    c->done = 1;
+    c->stop_reason = n == 0 ? kStopConverged : kStopMaximumRounds;
+  }
+}
+
+// Host-polled control: one round per launch (ordinary grid), then the
+// one-thread advance (jacobi_pull.hpp:153-220 shape: count the round, reset
+// the list that the next round fills, stop on an empty list or the limit).
+template <int B, int EPT>
+__global__ __launch_bounds__(B) void relax_round_kernel(DevGraph g, DevBatch b,
+                                                        std::uint32_t max_v,
+                                                        std::uint32_t work_words) {
+  extern __shared__ std::uint32_t lds_words[];
+  const Lds l = carve(lds_words, max_v, work_words);
+  DevControl* c = b.control;
+  if (load_agent(&c->done)) return;
+  const std::uint32_t r = load_agent(&c->round);
+  const std::uint32_t n = load_agent(&c->list_count[r % 3]);
+  process_round<B, EPT>(g, b, r, n, l);
+}
+
+__global__ void relax_advance_kernel(DevBatch b) {
+  DevControl* c = b.control;
+  if (c->done) return;
+  const std::uint32_t r = c->round;
+  const std::uint32_t n = c->list_count[r % 3];
+  if (b.instrumentation && n > c->max_list) c->max_list = n;
+  c->list_count[(r + 2) % 3] = 0;
+  c->claim[(r + 2) % 3] = 0;
+  c->round = r + 1;
+  c->rounds_executed = r + 1;
+  if (c->list_count[(r + 1) % 3] == 0) {
+    c->done = 1;
+    c->stop_reason = kStopConverged;
+  } else if (b.maximum_rounds && r + 1 >= b.maximum_rounds) {
+    c->done = 1;
+    c->stop_reason = kStopMaximumRounds;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batch preparation.
+// ---------------------------------------------------------------------------
+__global__ void reset_control_kernel(DevControl* c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    for (int i = 0; i < 3; ++i) {
+      c->list_count[i] = 0;
+      c->claim[i] = 0;
+    }
+    c->round = 0;
+    c->done = 0;
+    c->stop_reason = kStopNone;
+    c->error_bits = 0;
+    c->barrier_count = 0;
+    c->barrier_generation = 0;
+    c->rounds_executed = 0;
+    c->items_processed = 0;
+    c->edges_examined = 0;
+    c->atomic_attempts = 0;
+    c->atomic_successes = 0;
+    c->max_list = 0;
+  }
+}
+
+__global__ void fill_u32_kernel(std::uint32_t* p, unsigned long long n, std::uint32_t v) {
+  const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
+  for (unsigned long long i = static_cast<unsigned long long>(blockIdx.x) * blockDim.x +
+                              threadIdx.x;
+       i < n; i += stride)
+    p[i] = v;
+}
+
+// Resolves each slot's in/out neighbour tiles to the query's slots.
+__global__ void prep_slots_kernel(DevGraph g, DevBatch b) {
+  const std::uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= b.slot_count) return;
+  const std::uint32_t q = b.slot_query[s];
+  const std::uint32_t T = b.tiles[s];
+  const std::uint32_t qb = b.tile_offsets[q], qe = b.tile_offsets[q + 1];
+  const std::uint32_t in0 = g.in_nbr_offsets[T], in1 = g.in_nbr_offsets[T + 1];
+  const std::uint32_t ib = b.slot_in_base[s];
+  for (std::uint32_t k = in0; k < in1; ++k)
+    b.slot_in[ib + (k - in0)] = find_slot(b.tiles, qb, qe, g.in_nbr_tiles[k]);
+  const std::uint32_t o0 = g.out_nbr_offsets[T], o1 = g.out_nbr_offsets[T + 1];
+  const std::uint32_t obs = b.slot_out_base[s];
+  for (std::uint32_t k = o0; k < o1; ++k)
+    b.slot_out[obs + (k - o0)] = find_slot(b.tiles, qb, qe, g.out_nbr_tiles[k]);
+  b.stamp[s] = 0;
+  b.last_change[q] = 0;
+}
+
+// Zero the sources and queue round 0: Jacobi queues the source tiles and
+// every tile they feed; push engines queue the source tiles.
+__global__ void seed_kernel(DevGraph g, DevBatch b) {
+  const std::uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= b.query_count) return;
+  const std::uint32_t qb = b.tile_offsets[q], qe = b.tile_offsets[q + 1];
+  const bool jacobi = b.engine[q] == kJacobiPull;
+  for (std::uint32_t i = b.source_offsets[q]; i < b.source_offsets[q + 1]; ++i) {
+    const std::uint32_t u = b.sources[i];
+    const std::uint32_t T = g.owner_tiles[u];
+    const std::uint32_t s = find_slot(b.tiles, qb, qe, T);
+    if (s == kNone) {
+      atomicOr(&b.control->error_bits, 4u);  // device_failure: host validated this
+      continue;
+    }
+    const std::uint32_t v0 = g.tile_vertex_offsets[T];
+    const std::uint32_t npt = g.tile_vertex_offsets[T + 1] - v0;
+    std::uint32_t* column = b.dist + b.slot_dist[s];
+    column[u - v0] = 0u;
+    if (jacobi) column[npt + (u - v0)] = 0u;
+    enqueue(b, s, 0);
+    if (jacobi) {
+      const std::uint32_t ob = b.slot_out_base[s];
+      const std::uint32_t KO = g.out_nbr_offsets[T + 1] - g.out_nbr_offsets[T];
+      for (std::uint32_t k = 0; k < KO; ++k) {
+        const std::uint32_t ns = b.slot_out[ob + k];
+        if (ns != kNone) enqueue(b, ns, 0);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launch wrappers (host side of this translation unit).
+// ---------------------------------------------------------------------------
+template <int B, int EPT>
+hipError_t launch_relax(bool persistent, bool cooperative, std::uint32_t grid,
+                        std::size_t lds_bytes, std::uint32_t max_v, std::uint32_t work_words,
+                        const DevGraph& g, const DevBatch& b, hipStream_t stream) {
+  if (persistent) {
+    void* args[] = {const_cast<DevGraph*>(&g), const_cast<DevBatch*>(&b), &max_v, &work_words};
+    if (cooperative)
+      return hipLaunchCooperativeKernel(
+          reinterpret_cast<const void*>(&relax_persistent_kernel<B, EPT>), dim3(grid), dim3(B),
+          args, static_cast<unsigned>(lds_bytes), stream);
+    return hipLaunchKernel(reinterpret_cast<const void*>(&relax_persistent_kernel<B, EPT>),
+                           dim3(grid), dim3(B), args, lds_bytes, stream);
+  }
+  hipLaunchKernelGGL((relax_round_kernel<B, EPT>), dim3(grid), dim3(B), lds_bytes, stream, g, b,
+                     max_v, work_words);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(relax_advance_kernel, dim3(1), dim3(1), 0, stream, b);
+  return hipGetLastError();
+}
+
+template <int B, int EPT>
+hipError_t occupancy_relax(bool persistent, std::size_t lds_bytes, int* blocks_per_cu) {
+  if (persistent)
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        blocks_per_cu, reinterpret_cast<const void*>(&relax_persistent_kernel<B, EPT>), B,
+        lds_bytes);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      blocks_per_cu, reinterpret_cast<const void*>(&relax_round_kernel<B, EPT>), B, lds_bytes);
+}
+
+// Block sizes the engine is instantiated for, each with the registers per
+// thread that hold a tile's edge block (B * EPT edges per register load).
+#define BFN_FOR_EACH_SHAPE(X) X(64, 16) X(128, 16) X(256, 24) X(512, 12) X(1024, 8)
+
+hipError_t relax_launch(std::uint32_t block, bool persistent, bool cooperative,
+                        std::uint32_t grid, std::size_t lds_bytes, std::uint32_t max_v,
+                        std::uint32_t work_words, const DevGraph& g, const DevBatch& b,
+                        hipStream_t stream) {
+#define BFN_CASE(BS, E) \
+  if (block == BS)      \
+    return launch_relax<BS, E>(persistent, cooperative, grid, lds_bytes, max_v, work_words, g, b, stream);
+  BFN_FOR_EACH_SHAPE(BFN_CASE)
+#undef BFN_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t relax_occupancy(std::uint32_t block, bool persistent, std::size_t lds_bytes,
+                           int* blocks_per_cu) {
+#define BFN_CASE(BS, E) \
+  if (block == BS) return occupancy_relax<BS, E>(persistent, lds_bytes, blocks_per_cu);
+  BFN_FOR_EACH_SHAPE(BFN_CASE)
+#undef BFN_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t prep_launch(const DevGraph& g, const DevBatch& b, std::uint64_t pool_words,
+                       hipStream_t stream) {
+  hipLaunchKernelGGL(reset_control_kernel, dim3(1), dim3(64), 0, stream, b.control);
+  const std::uint32_t fill_grid = 4096;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(fill_grid), dim3(256), 0, stream, b.dist,
+                     static_cast<unsigned long long>(pool_words), kInfBits);
+  if (b.slot_count)
+    hipLaunchKernelGGL(prep_slots_kernel, dim3((b.slot_count + 255) / 256), dim3(256), 0,
+                       stream, g, b);
+  if (b.query_count)
+    hipLaunchKernelGGL(seed_kernel, dim3((b.query_count + 127) / 128), dim3(128), 0, stream, g,
+                       b);
+  return hipGetLastError();
+}
+
+}  // namespace bfn
+
+namespace bfn {
+hipError_t fill_launch(std::uint32_t* p, unsigned long long n, std::uint32_t value,
+                       hipStream_t stream) {
+  if (n) hipLaunchKernelGGL(fill_u32_kernel, dim3(2048), dim3(256), 0, stream, p, n, value);
+  return hipGetLastError();
+}
+}  // namespace bfn

@@ -1,0 +1,103 @@
+"""Shared test helpers: fixture materialisation and oracle wrappers."""
+import os
+import struct
+import sys
+
+import numpy as np
+
+import oracle
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+import make_fixtures as fixtures  # noqa: E402  (fixture data transcribed from the reference)
+
+SYNTHETIC, SYN_EDGE = 1, 1
+
+
+def fixture_arrays(case):
+    v = np.zeros(len(case["vertices"]), oracle.VERTEX_DTYPE)
+    for i, (x, y, loc, cls) in enumerate(case["vertices"]):
+        v[i] = (x, y, loc, cls)
+    e = np.zeros(len(case["edges"]), oracle.EDGE_DTYPE)
+    for i, (s, d, w, rec) in enumerate(case["edges"]):
+        e[i] = (s, d, w, SYNTHETIC, SYN_EDGE, 0, rec)
+    return v, e
+
+
+def core_case():
+    return dict(vertices=fixtures.CORE_VERTICES, edges=fixtures.CORE_EDGES, config=[0, 0, 64, 64])
+
+
+def spatial_case():
+    return dict(vertices=fixtures.SPATIAL_VERTICES, edges=fixtures.SPATIAL_EDGES,
+                config=[0, 0, 10, 10])
+
+
+def plain_vertices(count):
+    v = np.zeros(count, oracle.VERTEX_DTYPE)
+    for i in range(count):
+        v[i] = (i, 0, 1, 1)
+    return v
+
+
+def bits(a):
+    return np.asarray(a, np.float32).view(np.uint32)
+
+
+def f32_from_bits(values):
+    return np.asarray(values, np.uint32).view(np.float32)
+
+
+def fbits(x):
+    return struct.unpack("<I", struct.pack("<f", x))[0]
+
+
+def tile_mask(tile_count, tiles):
+    m = np.zeros(tile_count, np.uint8)
+    m[np.asarray(tiles, np.int64)] = 1
+    return m
+
+
+def oracle_query(view, sources, tiles):
+    """Oracle Dijkstra distances + relaxation count for one query over a
+    host-graph view (the product's CSR)."""
+    sel = tile_mask(view["tile_count"], tiles)
+    return oracle.dijkstra(view["vertex_count"], view["row_offsets"], view["destinations"],
+                           view["weights"], view["owner_tiles"], sel, sources)
+
+
+def certify_sssp(layout, sources, tiles, dist):
+    """Size-independent exactness certificate for strictly positive weights:
+    sources are 0, no admitted edge can lower a label (fixed point), and every
+    finite non-source label has a tight admitted in-edge from a finite label.
+    Following tight edges strictly decreases the label, so it ends at a
+    source: the labels are path sums no larger than any admitted path, i.e.
+    exactly the shortest-path fixed point.  Returns a description of the first
+    violation or ''."""
+    owner = layout["owner_tiles"]
+    sel = np.zeros(layout["tile_count"], bool)
+    sel[np.asarray(tiles, np.int64)] = True
+    vsel = sel[owner]
+    d = np.asarray(dist, np.float32)
+    if np.any(np.isfinite(d) & ~vsel):
+        return "finite label outside the selected tiles"
+    srcs = np.asarray(sources, np.int64)
+    if np.any(d[srcs] != 0):
+        return "source label is not zero"
+    col = layout["csc_column_offsets"].astype(np.int64)
+    dst = np.repeat(np.arange(len(col) - 1), np.diff(col))
+    src = layout["csc_sources"].astype(np.int64)
+    w = layout["csc_weights"]
+    adm = vsel[src] & vsel[dst]
+    cand = (d[src] + w).astype(np.float32)
+    if np.any(adm & (cand < d[dst])):
+        return "an admitted edge still relaxes (not a fixed point)"
+    tight = adm & np.isfinite(d[src]) & (cand == d[dst])
+    has = np.zeros(len(d), bool)
+    has[dst[tight]] = True
+    need = np.isfinite(d) & vsel
+    need[srcs] = False
+    if np.any(need & ~has):
+        return "a finite label has no tight admitted predecessor"
+    if np.any(w[adm] <= 0):
+        return "certificate needs strictly positive admitted weights"
+    return ""
