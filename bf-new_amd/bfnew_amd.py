@@ -154,6 +154,12 @@ class Response(C.Structure):
                 ("submit_ns", C.c_double), ("complete_ns", C.c_double)]
 
 
+class EngineCounters(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("tile_visits", "edges_relaxed", "algorithmic_bytes",
+                                          "atomic_attempts", "atomic_successes", "maximum_list",
+                                          "rounds_executed", "reserved")]
+
+
 VERTEX_DTYPE = np.dtype([("x", "<i4"), ("y", "<i4"), ("has_location", "<u4"),
                          ("resource_class", "<u4")])
 EDGE_DTYPE = np.dtype([("source", "<u4"), ("destination", "<u4"), ("weight", "<f4"),
@@ -193,6 +199,7 @@ HIP_SIGNATURES = {
     "bfn_batch_paths": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                          C.c_uint64, C.POINTER(C.c_uint64)], C.c_int32),
     "bfn_batch_occupancy": ([C.c_void_p, C.c_void_p], C.c_int32),
+    "bfn_batch_counters": ([C.c_void_p, C.POINTER(EngineCounters)], C.c_int32),
     "bfn_batch_timing": ([C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
                           C.POINTER(C.c_uint64)], C.c_int32),
     "bfn_server_create": ([C.c_void_p, C.c_void_p, C.POINTER(ServerOptions),
@@ -601,6 +608,11 @@ class Workspace:
         out = np.zeros(self.vertex_count, np.int32)
         _check(hip_lib().bfn_batch_occupancy(self.h, _ptr(out)))
         return out
+
+    def counters(self) -> EngineCounters:
+        c = EngineCounters()
+        _check(hip_lib().bfn_batch_counters(self.h, C.byref(c)))
+        return c
 
     def timing(self):
         r, t, n = C.c_float(), C.c_float(), C.c_uint64()

@@ -18,7 +18,7 @@
 //
 //   batch (one workspace; any number of queries)
 //     slot s = (query q, selected tile T); slots of q are q's selected tiles in
-//     sorted order starting at tile_offsets[q].
+//     sorted order, slots slot_begin[q] .. slot_end[q].
 //     slot_dist[s]   float offset of the slot's two npt-long copies in the pool:
 //                    Jacobi uses them as the two distance columns (read r&1,
 //                    write (r+1)&1); push engines keep distances in copy 0 and
@@ -96,6 +96,8 @@ struct DevControl {
   unsigned long long atomic_attempts;
   unsigned long long atomic_successes;
   unsigned long long max_list;
+  unsigned long long edges_relaxed;      // edge records evaluated by visits
+  unsigned long long algorithmic_bytes;  // modelled HBM bytes of the visits
 };
 
 struct DevBatch {
@@ -105,7 +107,8 @@ struct DevBatch {
   std::uint32_t tile_sweeps;      // Jacobi: 1 point, 0 closure
   std::uint32_t items_per_claim;
   std::uint32_t instrumentation;
-  const std::uint32_t* tile_offsets;    // [Q+1] slots of q
+  const std::uint32_t* slot_begin;      // [Q]   first slot of q
+  const std::uint32_t* slot_end;        // [Q]   one past q's last slot
   const std::uint32_t* tiles;           // [S]   tile of slot
   const std::uint32_t* source_offsets;  // [Q+1]
   const std::uint32_t* sources;
@@ -122,6 +125,7 @@ struct DevBatch {
   std::uint32_t* lists;                 // [3*S]
   std::uint32_t* dist;                  // float bits pool
   std::uint32_t* last_change;           // [Q] 1 + last round with a decrease
+  std::uint32_t* pending;               // [Q] queued work items of q, or null
   DevControl* control;
 };
 
@@ -157,6 +161,9 @@ struct PathArgs {
   std::uint32_t* arena_vertices;
   std::uint32_t* arena_labels;
   std::uint32_t* arena_edges;
+  const std::uint32_t* query_list;      // queries to reconstruct (null: all)
+  std::uint32_t jacobi_copy_from_control;  // take the column from round parity
+  std::uint32_t pad2;
 };
 
 // ---- memory helpers -------------------------------------------------------

@@ -2,8 +2,20 @@
 #pragma once
 
 #include "bfn_internal.hpp"
+#include "bfnew_amd.h"
+
+#include <string>
 
 namespace bfn {
+
+// Records the calling thread's last error (bfn_last_error) and returns code.
+int32_t set_error(int32_t code, const std::string& message);
+// Accessors of the handles defined in capi.hip, for the server.
+int device_ordinal(bfn_device device);
+std::size_t device_lds_limit(bfn_device device);
+std::uint32_t device_cu_count(bfn_device device);
+// Copies the graph's device view and the largest in/out neighbour-tile list.
+void graph_view(bfn_graph graph, DevGraph* view, std::uint32_t* max_in, std::uint32_t* max_out);
 
 struct PathArgs;
 

@@ -631,7 +631,8 @@ void bind_view(bfn_workspace ws, const BatchPlanHost& p, const std::size_t* seg,
   b.tile_sweeps = t->tile_sweeps;
   b.items_per_claim = std::max(1u, t->items_per_claim);
   b.instrumentation = o->instrumentation;
-  b.tile_offsets = reinterpret_cast<const std::uint32_t*>(m + seg[0]);
+  b.slot_begin = reinterpret_cast<const std::uint32_t*>(m + seg[0]);
+  b.slot_end = b.slot_begin + 1;
   b.tiles = reinterpret_cast<const std::uint32_t*>(m + seg[1]);
   b.source_offsets = reinterpret_cast<const std::uint32_t*>(m + seg[2]);
   b.sources = reinterpret_cast<const std::uint32_t*>(m + seg[3]);
@@ -648,6 +649,7 @@ void bind_view(bfn_workspace ws, const BatchPlanHost& p, const std::size_t* seg,
   b.lists = ws->lists.as<std::uint32_t>();
   b.dist = ws->pool.as<std::uint32_t>();
   b.last_change = ws->last_change.as<std::uint32_t>();
+  b.pending = nullptr;
   b.control = ws->control.as<DevControl>();
 }
 
@@ -1074,6 +1076,20 @@ int32_t bfn_batch_occupancy(bfn_workspace ws, int32_t* out) {
   return BFN_OK;
 }
 
+int32_t bfn_batch_counters(bfn_workspace ws, bfn_engine_counters* out) {
+  if (!ws || !ws->completed || !out) return fail(BFN_E_STATE, "batch has not completed");
+  const DevControl* c = ws->h_control.as<DevControl>();
+  std::memset(out, 0, sizeof *out);
+  out->tile_visits = c->items_processed;
+  out->edges_relaxed = c->edges_relaxed;
+  out->algorithmic_bytes = c->algorithmic_bytes;
+  out->atomic_attempts = c->atomic_attempts;
+  out->atomic_successes = c->atomic_successes;
+  out->maximum_list = c->max_list;
+  out->rounds_executed = c->rounds_executed;
+  return BFN_OK;
+}
+
 int32_t bfn_batch_timing(bfn_workspace ws, float* relax_ms, float* total_ms, uint64_t* launches) {
   if (!ws || !ws->completed) return fail(BFN_E_STATE, "batch has not completed");
   if (relax_ms) BFN_HIP(hipEventElapsedTime(relax_ms, ws->ev_relax0, ws->ev_relax1));
@@ -1082,3 +1098,17 @@ int32_t bfn_batch_timing(bfn_workspace ws, float* relax_ms, float* total_ms, uin
   return BFN_OK;
 }
 
+
+namespace bfn {
+int32_t set_error(int32_t code, const std::string& message) { return fail(code, message); }
+int device_ordinal(bfn_device device) { return device->ordinal; }
+std::size_t device_lds_limit(bfn_device device) { return device->props.sharedMemPerBlock; }
+std::uint32_t device_cu_count(bfn_device device) {
+  return static_cast<std::uint32_t>(device->props.multiProcessorCount);
+}
+void graph_view(bfn_graph graph, DevGraph* view, std::uint32_t* max_in, std::uint32_t* max_out) {
+  *view = graph->view;
+  *max_in = graph->info.max_in_neighbor_tiles;
+  *max_out = graph->info.max_out_neighbor_tiles;
+}
+}  // namespace bfn

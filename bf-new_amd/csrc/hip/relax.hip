@@ -62,6 +62,7 @@ __device__ __forceinline__ void enqueue(const DevBatch& b, std::uint32_t slot,
     const std::uint32_t li = round % 3;
     const std::uint32_t pos = atomic_add_agent(&b.control->list_count[li], 1u);
     b.lists[static_cast<std::size_t>(li) * b.slot_count + pos] = slot;
+    if (b.pending) atomic_add_agent(&b.pending[b.slot_query[slot]], 1u);
   }
 }
 
@@ -98,7 +99,8 @@ __device__ __forceinline__ std::uint32_t pk_dst(std::uint32_t p) { return p & kP
 // ---------------------------------------------------------------------------
 template <int B, int EPT>
 __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b, std::uint32_t s,
-                           std::uint32_t r, const Lds& l, EdgeRegs<B, EPT>& regs) {
+                           std::uint32_t r, const Lds& l, EdgeRegs<B, EPT>& regs,
+                           unsigned long long& bytes, unsigned long long& edges) {
   const std::uint32_t tid = threadIdx.x;
   const std::uint32_t T = b.tiles[s];
   const std::uint32_t q = b.slot_query[s];
@@ -120,6 +122,7 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
     if (ns == kNone) {
       for (std::uint32_t i = tid; i < n; i += B) dst[i] = kInfBits;
     } else {
+      if (tid == 0) bytes += 4ull * n;
       const std::uint32_t* src = b.dist + b.slot_dist[ns] + static_cast<std::size_t>(rp) * n;
       for (std::uint32_t i = tid; i < n; i += B) dst[i] = src[i];
     }
@@ -131,6 +134,11 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
   const std::uint32_t e0 = g.column_offsets[v0];
   const std::uint32_t e1 = g.column_offsets[v0 + npt];
   const bool single = (e1 - e0) <= static_cast<std::uint32_t>(B * EPT);
+  if (tid == 0) {
+    // edge records (pack + weight) once, the written column once
+    bytes += 8ull * (e1 - e0) + 4ull * npt;
+    edges += e1 - e0;
+  }
   // Pass 1: every admitted in-edge reads the staged read column.
   for (std::uint32_t base = e0; base < e1; base += B * EPT) {
     load_edges<B, EPT>(regs, g.csc_pack, g.in_weights, base, e1);
@@ -192,7 +200,8 @@ template <int B, int EPT>
 __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b, std::uint32_t s,
                            std::uint32_t r, bool frontier, const Lds& l,
                            EdgeRegs<B, EPT>& regs, unsigned long long& attempts,
-                           unsigned long long& successes) {
+                           unsigned long long& successes, unsigned long long& bytes,
+                           unsigned long long& edges) {
   const std::uint32_t tid = threadIdx.x;
   const std::uint32_t T = b.tiles[s];
   const std::uint32_t q = b.slot_query[s];
@@ -220,6 +229,11 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
   const std::uint32_t e0 = g.row_offsets[v0];
   const std::uint32_t e1 = g.row_offsets[v0 + npt];
   const bool single = (e1 - e0) <= static_cast<std::uint32_t>(B * EPT);
+  if (tid == 0) {
+    // edge records once, the column (and the snapshot for frontier) once
+    bytes += 8ull * (e1 - e0) + 4ull * npt * (frontier ? 2 : 1);
+    edges += e1 - e0;
+  }
   if (single) load_edges<B, EPT>(regs, g.csr_pack, g.weights, e0, e1);
   // Internal edges to the tile's local fixed point, from frontier sources.
   for (;;) {
@@ -268,6 +282,8 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
   }
   attempts += att;
   successes += suc;
+  // remote label read per attempt, read-modify-write per atomic issued
+  bytes += 4ull * att + 8ull * suc;
   bool changed = false;
   for (std::uint32_t i = tid; i < npt; i += B) {
     const std::uint32_t v = l.own[i];
@@ -299,7 +315,7 @@ __device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch&
   const std::uint32_t li = r % 3;
   const std::uint32_t chunk = b.items_per_claim ? b.items_per_claim : 1u;
   EdgeRegs<B, EPT> regs;
-  unsigned long long attempts = 0, successes = 0, items = 0;
+  unsigned long long attempts = 0, successes = 0, items = 0, bytes = 0, edges = 0;
   for (;;) {
     if (threadIdx.x == 0) l.flags[0] = atomic_add_agent(&b.control->claim[li], chunk);
     __syncthreads();
@@ -311,12 +327,15 @@ __device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch&
       const std::uint32_t s = b.lists[static_cast<std::size_t>(li) * b.slot_count + i];
       const std::uint32_t engine = b.engine[b.slot_query[s]];
       if (engine == kJacobiPull) {
-        visit_pull<B, EPT>(g, b, s, r, l, regs);
+        visit_pull<B, EPT>(g, b, s, r, l, regs, bytes, edges);
       } else {
-        visit_push<B, EPT>(g, b, s, r, engine == kFrontierPush, l, regs, attempts, successes);
+        visit_push<B, EPT>(g, b, s, r, engine == kFrontierPush, l, regs, attempts, successes,
+                           bytes, edges);
       }
       ++items;
       __syncthreads();
+      // the visit's own enqueues are issued before its item is retired
+      if (b.pending && threadIdx.x == 0) atomic_add_agent(&b.pending[b.slot_query[s]], 0xFFFFFFFFu);
     }
   }
   if (b.instrumentation) {
@@ -324,6 +343,8 @@ __device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch&
     if (attempts) atomicAdd(&b.control->atomic_attempts, attempts);
     if (successes) atomicAdd(&b.control->atomic_successes, successes);
     if (threadIdx.x == 0 && items) atomicAdd(&b.control->items_processed, items);
+    if (bytes) atomicAdd(&b.control->algorithmic_bytes, bytes);
+    if (edges) atomicAdd(&b.control->edges_relaxed, edges);
   }
 }
 
@@ -433,6 +454,8 @@ __global__ void reset_control_kernel(DevControl* c) {
     c->atomic_attempts = 0;
     c->atomic_successes = 0;
     c->max_list = 0;
+    c->edges_relaxed = 0;
+    c->algorithmic_bytes = 0;
   }
 }
 
@@ -450,7 +473,7 @@ __global__ void prep_slots_kernel(DevGraph g, DevBatch b) {
   if (s >= b.slot_count) return;
   const std::uint32_t q = b.slot_query[s];
   const std::uint32_t T = b.tiles[s];
-  const std::uint32_t qb = b.tile_offsets[q], qe = b.tile_offsets[q + 1];
+  const std::uint32_t qb = b.slot_begin[q], qe = b.slot_end[q];
   const std::uint32_t in0 = g.in_nbr_offsets[T], in1 = g.in_nbr_offsets[T + 1];
   const std::uint32_t ib = b.slot_in_base[s];
   for (std::uint32_t k = in0; k < in1; ++k)
@@ -468,7 +491,7 @@ __global__ void prep_slots_kernel(DevGraph g, DevBatch b) {
 __global__ void seed_kernel(DevGraph g, DevBatch b) {
   const std::uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= b.query_count) return;
-  const std::uint32_t qb = b.tile_offsets[q], qe = b.tile_offsets[q + 1];
+  const std::uint32_t qb = b.slot_begin[q], qe = b.slot_end[q];
   const bool jacobi = b.engine[q] == kJacobiPull;
   for (std::uint32_t i = b.source_offsets[q]; i < b.source_offsets[q + 1]; ++i) {
     const std::uint32_t u = b.sources[i];

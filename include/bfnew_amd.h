@@ -276,6 +276,22 @@ int32_t bfn_batch_occupancy(bfn_workspace workspace, int32_t *out);
 int32_t bfn_batch_timing(bfn_workspace workspace, float *relax_ms, float *total_ms,
                          uint64_t *launches);
 
+/* Engine counters of the last batch, collected when the run options ask for
+ * instrumentation (light or debug); zero otherwise. */
+typedef struct {
+  uint64_t tile_visits;          /* (query, tile) work items processed */
+  uint64_t edges_relaxed;        /* edge records evaluated by those visits */
+  uint64_t algorithmic_bytes;    /* modelled HBM bytes: edge records, staged and
+                                    written label slabs, remote label reads and
+                                    atomics */
+  uint64_t atomic_attempts;
+  uint64_t atomic_successes;
+  uint64_t maximum_list;
+  uint64_t rounds_executed;
+  uint64_t reserved;
+} bfn_engine_counters;
+int32_t bfn_batch_counters(bfn_workspace workspace, bfn_engine_counters *out);
+
 /* Serving loop (continuous batching, north of the reference's single-batch
  * engines): a server owns a thread and a stream over a resident graph.
  * Requests are admitted by free workspace memory and join the running

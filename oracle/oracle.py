@@ -301,6 +301,29 @@ def ref_lib():
     return _ref
 
 
+def ref_csr_sssp(V, row_offsets, destinations, weights, owner_tiles, tile_selected, sources):
+    """The reference's dijkstra_oracle on the induced subgraph (built by the
+    reference's build_weighted_graph, untimed).  Returns (ns, relaxations)."""
+    lib = ref_lib()
+    if not hasattr(lib, "_csr_bound"):
+        P = C.c_void_p
+        lib.ref_csr_sssp.argtypes = [C.c_uint32, P, P, P, P, P, P, C.c_uint32, P]
+        lib.ref_csr_sssp.restype = C.c_int64
+        lib._csr_bound = True
+    ro = np.ascontiguousarray(row_offsets, np.uint64)
+    de = np.ascontiguousarray(destinations, np.uint32)
+    w = np.ascontiguousarray(weights, np.float32)
+    own = np.ascontiguousarray(owner_tiles, np.uint32)
+    sel = np.ascontiguousarray(tile_selected, np.uint8)
+    src = np.ascontiguousarray(sources, np.uint32)
+    relax = C.c_uint64(0)
+    ns = lib.ref_csr_sssp(V, _p(ro), _p(de), _p(w), _p(own), _p(sel), _p(src), len(src),
+                          C.byref(relax))
+    if ns < 0:
+        raise ValueError(f"reference csr sssp failed: {ns}")
+    return ns, relax.value
+
+
 class RefGraph:
     """Handle over the reference's build_spatially_ordered_graph result."""
 

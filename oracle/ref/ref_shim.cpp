@@ -236,6 +236,61 @@ std::int64_t ref_sssp(void* handle, const std::uint8_t* tile_selected,
   }
 }
 
+// The reference CPU path on one query of a graph given as CSR arrays (the
+// product's synthetic layout): the induced subgraph of the selected tiles
+// (query.cpp:286-362 restated: kept vertices in global order, edges with both
+// endpoints kept, provenance = CSR position so canonical order is kept) is
+// built by the reference's build_weighted_graph, then dijkstra_oracle runs
+// timed.  Returns the Dijkstra wall time in ns (<0 on error); relaxations
+// receives the admitted edges whose source is reachable.
+std::int64_t ref_csr_sssp(std::uint32_t vertex_count, const std::uint64_t* row_offsets,
+                          const std::uint32_t* destinations, const float* weights,
+                          const std::uint32_t* owner_tiles, const std::uint8_t* tile_selected,
+                          const std::uint32_t* sources, std::uint32_t source_count,
+                          std::uint64_t* relaxations) {
+  try {
+    constexpr std::uint32_t unmapped = std::numeric_limits<std::uint32_t>::max();
+    std::vector<std::uint32_t> local(vertex_count, unmapped);
+    std::uint32_t kept = 0;
+    for (std::uint32_t v = 0; v < vertex_count; ++v)
+      if (tile_selected[owner_tiles[v]]) local[v] = kept++;
+    std::vector<bfnew::VertexMetadata> vertices(kept,
+                                                bfnew::VertexMetadata::unlocated(bfnew::ResourceClassId{1}));
+    std::vector<bfnew::EdgeInputRecord> edges;
+    for (std::uint32_t s = 0; s < vertex_count; ++s) {
+      if (local[s] == unmapped) continue;
+      for (std::uint64_t p = row_offsets[s]; p < row_offsets[s + 1]; ++p) {
+        const std::uint32_t d = destinations[p];
+        if (local[d] == unmapped) continue;
+        edges.push_back(bfnew::EdgeInputRecord{
+            bfnew::VertexId{local[s]}, bfnew::VertexId{local[d]}, weights[p],
+            bfnew::PhysicalProvenance{1U, 1U, p}});
+      }
+    }
+    bfnew::InputGraph input(std::move(vertices), std::move(edges));
+    const bfnew::WeightedGraph induced = bfnew::build_weighted_graph(input);
+    std::vector<bfnew::VertexId> local_sources;
+    for (std::uint32_t i = 0; i < source_count; ++i) {
+      if (sources[i] >= vertex_count || local[sources[i]] == unmapped) return -1;
+      local_sources.push_back(bfnew::VertexId{local[sources[i]]});
+    }
+    const auto start = std::chrono::steady_clock::now();
+    const bfnew::SsspResult result = bfnew::dijkstra_oracle(induced, local_sources);
+    const auto stop = std::chrono::steady_clock::now();
+    if (relaxations != nullptr) {
+      std::uint64_t count = 0;
+      const auto out = induced.outgoing();
+      for (std::uint32_t s = 0; s < induced.vertex_count(); ++s)
+        if (result.distances[s] != std::numeric_limits<float>::infinity())
+          count += out.row_offsets[s + 1] - out.row_offsets[s];
+      *relaxations = count;
+    }
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(stop - start).count();
+  } catch (const std::exception&) {
+    return -2;
+  }
+}
+
 // reconstruct_path_from_distances on the full graph (no tile restriction),
 // for the sssp_test fixtures.  Returns the path length in edges, -1 when the
 // target is unreachable, -2 on error.

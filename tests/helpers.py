@@ -83,10 +83,14 @@ def certify_sssp(layout, sources, tiles, dist):
     srcs = np.asarray(sources, np.int64)
     if np.any(d[srcs] != 0):
         return "source label is not zero"
-    col = layout["csc_column_offsets"].astype(np.int64)
-    dst = np.repeat(np.arange(len(col) - 1), np.diff(col))
-    src = layout["csc_sources"].astype(np.int64)
+    if "_csc_dst" not in layout:
+        col = layout["csc_column_offsets"].astype(np.int64)
+        layout["_csc_dst"] = np.repeat(np.arange(len(col) - 1, dtype=np.int64), np.diff(col))
+        layout["_csc_src"] = layout["csc_sources"].astype(np.int64)
+    dst = layout["_csc_dst"]
+    src = layout["_csc_src"]
     w = layout["csc_weights"]
+    # restrict to edges whose destination tile is selected (contiguous spans)
     adm = vsel[src] & vsel[dst]
     cand = (d[src] + w).astype(np.float32)
     if np.any(adm & (cand < d[dst])):

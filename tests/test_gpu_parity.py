@@ -1,0 +1,282 @@
+"""GPU parity: every engine and control mode through the C ABI against the
+oracle and the golden vectors (bit-exact distances, reference round counts,
+predecessors and compact paths)."""
+import itertools
+
+import numpy as np
+import pytest
+
+import bfnew_amd as bfn
+import oracle
+from helpers import bits, certify_sssp, fbits, fixture_arrays, oracle_query, tile_mask
+
+pytestmark = pytest.mark.gpu
+
+ENGINES = (bfn.JACOBI_PULL, bfn.DENSE_CHAOTIC_PUSH, bfn.FRONTIER_PUSH)
+CONTROLS = (bfn.PERSISTENT, bfn.CHUNKED, bfn.PER_ROUND)
+
+
+def configurations():
+    for eng, ctl in itertools.product(ENGINES, CONTROLS):
+        for sweeps in ((1, 0) if eng == bfn.JACOBI_PULL else (1,)):
+            yield eng, ctl, sweeps
+
+
+def fixture_graph(device, case):
+    v, e = fixture_arrays(case)
+    hg = bfn.HostGraph.build(v, e, case["config"])
+    return hg, bfn.DeviceGraph(device, hg.layout())
+
+
+def run(device, dg, batch, engine, control, sweeps=1, max_rounds=1 << 30, block=256, **kw):
+    ws = bfn.Workspace(device)
+    ws.submit(dg, batch, bfn.run_options(engine=engine, control_mode=control, rounds_per_chunk=3,
+                                         maximum_rounds=max_rounds, block_size=block),
+              bfn.tuning(tile_sweeps=sweeps, **kw))
+    return ws
+
+
+def test_fixture_suite_all_engines(device, golden):
+    for case in golden["engine_cases"]:
+        hg, dg = fixture_graph(device, case)
+        view = hg.view()
+        exp = case["expected"]
+        src = [int(view["old_to_new"][s]) for s in case["sources"]]
+        tgt = [int(view["old_to_new"][t]) for t in case["targets"]]
+        batch = bfn.Batch(sources=[src], targets=[tgt], tiles=[case["selected_tiles"]])
+        for eng, ctl, sweeps in configurations():
+            label = (case["name"], eng, ctl, sweeps)
+            ws = run(device, dg, batch, eng, ctl, sweeps)
+            st, per, _ = ws.status()
+            assert st.converged == 1 and per[0].converged == 1, label
+            d = ws.distances(0)
+            assert list(bits(d)) == exp["distance_bits"], label
+            if eng == bfn.JACOBI_PULL and sweeps == 1:
+                # point Jacobi keeps the reference's round count
+                assert per[0].rounds_completed == exp["jacobi_rounds"], label
+                assert st.rounds_completed == exp["jacobi_rounds"], label
+            assert per[0].reached_all_targets == exp["reached"], label
+            assert per[0].bounding_box_miss == 1 - exp["reached"], label
+            if ctl == bfn.PER_ROUND:
+                assert list(ws.predecessors(0)) == exp["predecessors"], label
+                paths = ws.paths()
+                for p in paths:
+                    g = exp["paths"][str(p["target"])]
+                    if g["status"] == "complete":
+                        assert p["reconstruction"] == 0 and p["reached"] == 1, label
+                        assert list(p["vertices"]) == g["vertices"], label
+                        assert list(p["edge_ids"]) == g["edge_ids"], label
+                        assert p["path_length"] == len(g["edge_ids"])
+                        assert list(bits(p["labels"])) == [exp["distance_bits"][x] for x in g["vertices"]]
+                    else:
+                        assert p["reconstruction"] == 1 and p["reached"] == 0, label
+            ws.close()
+        dg.close()
+
+
+def test_reference_round_literals(device, golden):
+    # tests/jacobi_fixture_suite.hpp:195-355 literal round counts
+    for case in golden["engine_cases"]:
+        if "expected_rounds" not in case:
+            continue
+        hg, dg = fixture_graph(device, case)
+        view = hg.view()
+        src = [int(view["old_to_new"][s]) for s in case["sources"]]
+        tgt = [int(view["old_to_new"][t]) for t in case["targets"]]
+        ws = run(device, dg, bfn.Batch([src], [tgt], [case["selected_tiles"]]), bfn.JACOBI_PULL,
+                 bfn.PER_ROUND)
+        st, per, _ = ws.status()
+        assert st.rounds_completed == case["expected_rounds"], case["name"]
+        if "expected_bounded_target" in case:
+            d = ws.distances(0)
+            assert bits(d)[tgt[0]] == fbits(case["expected_bounded_target"])
+            assert st.reached_target_mask == case["expected_reached"]
+            assert st.bounding_box_miss_mask == case["expected_miss"]
+
+
+def test_maximum_rounds_exhaustion(device, golden):
+    case = next(c for c in golden["engine_cases"] if c["name"].startswith("long_chain"))
+    hg, dg = fixture_graph(device, case)
+    batch = bfn.Batch([[0]], [[1]], [case["selected_tiles"]])
+    for ctl in CONTROLS:
+        ws = run(device, dg, batch, bfn.JACOBI_PULL, ctl, max_rounds=2)
+        st, per, _ = ws.status()
+        assert st.stop_reason == 2 and st.converged == 0 and st.rounds_completed == 2, ctl
+        assert st.reached_target_mask == 0 and st.bounding_box_miss_mask == 0
+        assert per[0].converged == 0
+        paths = ws.paths()
+        assert paths[0]["reconstruction"] == 2  # query_terminal_failure
+
+
+def test_batched_mixed_duration_lanes(device, golden):
+    bc = golden["batched"]
+    hg, dg = fixture_graph(device, bc)
+    view = hg.view()
+    lanes = bc["lanes"]
+    batch = bfn.Batch(sources=[[int(view["old_to_new"][s]) for s in l["sources"]] for l in lanes],
+                      targets=[[int(view["old_to_new"][t]) for t in l["targets"]] for l in lanes],
+                      tiles=[l["selected_tiles"] for l in lanes])
+    for eng, ctl, sweeps in configurations():
+        ws = run(device, dg, batch, eng, ctl, sweeps)
+        st, per, _ = ws.status()
+        for q, lane in enumerate(lanes):
+            assert list(bits(ws.distances(q))) == lane["distance_bits"], (eng, ctl, q)
+            if eng == bfn.JACOBI_PULL and sweeps == 1:
+                assert per[q].rounds_completed == lane["jacobi_rounds"], (ctl, q)
+            assert per[q].reached_all_targets == lane["reached"]
+        if eng == bfn.JACOBI_PULL and sweeps == 1:
+            assert st.rounds_completed == max(l["jacobi_rounds"] for l in lanes)
+        assert st.valid_lane_mask == 0b11111
+        ws.close()
+
+
+def test_compact_path_fixture(device, golden):
+    cc = golden["compact"]
+    hg, dg = fixture_graph(device, cc)
+    view = hg.view()
+    qs = cc["queries"]
+    o2n = view["old_to_new"]
+    batch = bfn.Batch(sources=[[int(o2n[s]) for s in q["sources"]] for q in qs],
+                      targets=[[int(o2n[t]) for t in q["targets"]] for q in qs],
+                      tiles=[q["selected_tiles"] for q in qs])
+    for eng in ENGINES:
+        ws = run(device, dg, batch, eng, bfn.PER_ROUND)
+        paths = ws.paths()
+        by = {(qi, p["target"]): p for qi, q in enumerate(qs) for p in
+              paths[batch.target_offsets[qi]:batch.target_offsets[qi + 1]]}
+        for qi, q in enumerate(qs):
+            assert list(bits(ws.distances(qi))) == q["distance_bits"]
+            for old_t, lit in q["expected_paths"].items():
+                p = by[(qi, int(o2n[int(old_t)]))]
+                if lit is None:
+                    assert p["reconstruction"] == 1 and p["reached"] == 0
+                    assert p["has_selected_source"] == 0 and np.isinf(p["distance"])
+                    continue
+                assert p["reconstruction"] == 0 and p["has_selected_source"] == 1
+                assert [int(view["new_to_old"][x]) for x in p["vertices"]] == lit["vertices"]
+                assert [float(x) for x in p["labels"]] == lit["labels"]
+                assert p["selected_source"] == p["vertices"][0]
+                g = q["paths"][str(int(o2n[int(old_t)]))]
+                assert list(p["edge_ids"]) == g["edge_ids"]
+        ws.close()
+
+
+def synthetic(device, tx, ty, npt, seed, host_graph=True):
+    hg = bfn.HostGraph.synthetic(tx, ty, npt, seed, host_graph=host_graph)
+    return hg, bfn.DeviceGraph(device, hg.layout())
+
+
+def test_c1_full_graph_distances_and_predecessors(device):
+    hg, dg = synthetic(device, 16, 16, 64, 1)
+    view = hg.view()
+    T = view["tile_count"]
+    src, tgt = 4321, 9876
+    ref, _ = oracle.dijkstra(view["vertex_count"], view["row_offsets"], view["destinations"],
+                             view["weights"], view["owner_tiles"], None, [src])
+    pv, _ = oracle.predecessors(view["vertex_count"], view["column_offsets"], view["sources"],
+                                view["in_weights"], view["in_edge_ids"], view["owner_tiles"], None,
+                                ref, [src])
+    batch = bfn.Batch([[src]], [[tgt]], [list(range(T))])
+    for eng, ctl, sweeps in configurations():
+        ws = run(device, dg, batch, eng, ctl, sweeps)
+        assert np.array_equal(bits(ws.distances(0)), bits(ref)), (eng, ctl, sweeps)
+        assert np.array_equal(ws.predecessors(0), pv)
+        p = ws.paths()[0]
+        st, ov, oe = oracle.reconstruct(view["vertex_count"], view["column_offsets"],
+                                        view["sources"], view["in_weights"], view["in_edge_ids"],
+                                        view["owner_tiles"], None, ref, [src], tgt)
+        assert list(p["vertices"]) == list(ov) and list(p["edge_ids"]) == list(oe)
+        ws.close()
+
+
+def c2_batch(hg, count, seed=2):
+    qs = hg.synthetic_queries(count, seed=seed, side_min=4, side_max=32)
+    tiles = [hg.query_tiles(q) for q in qs]
+    return qs, bfn.Batch([[q.source] for q in qs], [[q.target] for q in qs], tiles)
+
+
+def test_c2_batched_and_plain_bit_exact(device):
+    hg, dg = synthetic(device, 64, 64, 256, 2)
+    view = hg.view()
+    qs, batch = c2_batch(hg, 24)
+    refs = [oracle_query(view, [q.source], batch.tiles[i])[0] for i, q in enumerate(qs)]
+    for eng, sweeps in ((0, 0), (0, 1), (1, 1), (2, 1)):
+        ws = run(device, dg, batch, eng, bfn.PERSISTENT, sweeps)
+        for i in range(len(qs)):
+            assert np.array_equal(bits(ws.distances(i)), bits(refs[i])), (eng, sweeps, i)
+        ws.close()
+    # plain mode: one query per launch
+    for i in range(4):
+        one = bfn.Batch([[qs[i].source]], [[qs[i].target]], [batch.tiles[i]])
+        for eng in ENGINES:
+            ws = run(device, dg, one, eng, bfn.PER_ROUND)
+            assert np.array_equal(bits(ws.distances(0)), bits(refs[i]))
+            ws.close()
+
+
+def test_mixed_engines_and_rerun(device):
+    hg, dg = synthetic(device, 32, 32, 128, 4)
+    view = hg.view()
+    qs, batch = c2_batch(hg, 12, seed=5)
+    batch.engines = [i % 3 for i in range(len(qs))]
+    refs = [oracle_query(view, [q.source], batch.tiles[i])[0] for i, q in enumerate(qs)]
+    ws = run(device, dg, batch, bfn.JACOBI_PULL, bfn.PERSISTENT, 0)
+    first = [ws.distances(i) for i in range(len(qs))]
+    for i in range(len(qs)):
+        assert np.array_equal(bits(first[i]), bits(refs[i])), i
+    for _ in range(3):
+        ws.rerun()
+        for i in range(len(qs)):
+            assert np.array_equal(bits(ws.distances(i)), bits(first[i]))
+    _, per, _ = ws.status()
+    assert [p.engine for p in per] == batch.engines
+
+
+def test_hipgraph_replay(device):
+    hg, dg = synthetic(device, 32, 32, 128, 6)
+    view = hg.view()
+    qs, batch = c2_batch(hg, 8, seed=7)
+    refs = [oracle_query(view, [q.source], batch.tiles[i])[0] for i, q in enumerate(qs)]
+    for eng in ENGINES:
+        ws = run(device, dg, batch, eng, bfn.CHUNKED, 0, use_graph=1, graph_rounds=3)
+        for rep in range(3):
+            for i in range(len(qs)):
+                assert np.array_equal(bits(ws.distances(i)), bits(refs[i])), (eng, rep, i)
+            ws.rerun()
+        ws.close()
+
+
+def test_relaxation_counts_match_oracle(device):
+    hg, dg = synthetic(device, 32, 32, 128, 8)
+    view = hg.view()
+    qs, batch = c2_batch(hg, 6, seed=9)
+    ws = run(device, dg, batch, bfn.JACOBI_PULL, bfn.PERSISTENT, 0, count_relaxations=1)
+    _, per, _ = ws.status()
+    for i, q in enumerate(qs):
+        _, relax = oracle_query(view, [q.source], batch.tiles[i])
+        assert per[i].relaxations == relax
+
+
+def test_c3_batch_certificate(device):
+    """Config 3 at full size: 256 queries in one batched persistent launch.
+    Exactness for all queries through the size-independent certificate, and
+    bit-exact agreement with the oracle on a sample."""
+    hg, dg = synthetic(device, 128, 128, 512, 3, host_graph=False)
+    lay = hg.layout_arrays()
+    qs = hg.synthetic_queries(256, seed=33, side_min=4, side_max=64, unbounded_fraction=0.01)
+    tiles = [hg.query_tiles(q) for q in qs]
+    batch = bfn.Batch([[q.source] for q in qs], [[q.target] for q in qs], tiles)
+    ws = run(device, dg, batch, bfn.JACOBI_PULL, bfn.PERSISTENT, 0)
+    st, per, _ = ws.status()
+    assert st.converged == 1
+    for i in list(range(0, 256, 16)) + [i for i, q in enumerate(qs) if q.unbounded][:2]:
+        d = ws.distances(i)
+        assert certify_sssp(lay, [qs[i].source], tiles[i], d) == "", i
+    view = dict(vertex_count=lay["vertex_count"], tile_count=lay["tile_count"],
+                row_offsets=lay["csr_row_offsets"].astype(np.uint64),
+                destinations=lay["csr_destinations"], weights=lay["csr_weights"],
+                owner_tiles=lay["owner_tiles"])
+    small = sorted(range(256), key=lambda i: len(tiles[i]))[:4]
+    for i in small:
+        ref, _ = oracle_query(view, [qs[i].source], tiles[i])
+        assert np.array_equal(bits(ws.distances(i)), bits(ref)), i
