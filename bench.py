@@ -69,7 +69,7 @@ def make_batch(hg, cfg):
 
 def apply_tuning(batch, qs, tune_entry):
     t = dict(engine=bfn.JACOBI_PULL, block=256, sweeps=0, control=bfn.PERSISTENT,
-             classes=None, items_per_claim=1)
+             classes=None, items_per_claim=1, use_graph=0, graph_rounds=0)
     if tune_entry:
         t.update(tune_entry)
     if t.get("classes"):
@@ -103,7 +103,7 @@ def cpu_baseline(hg_view, qs, tiles, budget_s):
     # untimed induced-graph construction of the reference builder.
     order = [i for i in sorted(range(len(qs)), key=lambda i: len(tiles[i]))
              if len(tiles[i]) <= 1024]
-    picked = order[::max(1, len(order) // 24)]
+    picked = order[::max(1, len(order) // 64)]
     wall = time.perf_counter()
     for i in picked:
         if time.perf_counter() - wall > 4 * budget_s:
@@ -169,6 +169,19 @@ def plain_vs_batched(dev, steps):
                 batched_over_best_plain=batched / best_plain)
 
 
+def copy_calibration():
+    """Three copies of 1 GiB between two device buffers: a kernel of known
+    traffic (1 GiB read, 1 GiB written per dispatch) against which the HBM
+    FETCH_SIZE / WRITE_SIZE counters are scaled."""
+    import torch
+    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("copy calibration: 3 x 1073741824 bytes copied")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,7 +189,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--cpu-budget-s", type=float, default=12.0)
+    ap.add_argument("--cpu-budget-s", type=float, default=10.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--engine", type=int, default=None, help="override the tuned engine")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -185,6 +200,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        copy_calibration()
+        return
     dev = bfn.Device(0)
     info = dev.info()
     log("device", info.arch.decode(), info.compute_units, "CUs")
@@ -206,7 +224,8 @@ def main():
     ws.submit(dg, batch, bfn.run_options(engine=tune["engine"], control_mode=tune["control"],
                                          block_size=tune["block"], instrumentation=1),
               bfn.tuning(tile_sweeps=tune["sweeps"], count_relaxations=1,
-                         items_per_claim=tune["items_per_claim"]))
+                         items_per_claim=tune["items_per_claim"], use_graph=tune["use_graph"],
+                         graph_rounds=tune["graph_rounds"]))
     st, per, _ = ws.status()
     if not st.converged:
         raise SystemExit("batch did not converge")
@@ -214,7 +233,9 @@ def main():
     counters = ws.counters()
     # Timed steps with instrumentation off.
     ws.submit(dg, batch, opts, bfn.tuning(tile_sweeps=tune["sweeps"],
-                                          items_per_claim=tune["items_per_claim"]))
+                                          items_per_claim=tune["items_per_claim"],
+                                          use_graph=tune["use_graph"],
+                                          graph_rounds=tune["graph_rounds"]))
     elapsed, relax_ms = timed_steps(ws, args.steps, args.warmup)
     value = relax_per_step * args.steps / elapsed
     ms_per_step = elapsed * 1e3 / args.steps
@@ -240,7 +261,9 @@ def main():
                    "queries": C3["queries"], "box_side": list(C3["side"]),
                    "unbounded_fraction": C3["unbounded"], "engine": bfn.ENGINE_NAMES[tune["engine"]],
                    "per_class_engines": tune.get("classes"), "block": tune["block"],
-                   "tile_sweeps": tune["sweeps"], "control": bfn.CONTROL_NAMES[tune["control"]],
+                   "tile_sweeps": tune["sweeps"],
+                   "control": bfn.CONTROL_NAMES[tune["control"]] +
+                   (f"+hipgraph({tune['graph_rounds']} rounds)" if tune["use_graph"] else ""),
                    "relaxations_per_step": relax_per_step},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

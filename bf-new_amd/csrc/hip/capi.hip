@@ -719,12 +719,14 @@ int32_t run_resident_batch(bfn_workspace ws) {
     ++ws->host_syncs;
     ++ws->controller_copies;
     DevControl* hc = ws->h_control.as<DevControl>();
-    // Continue host-polled when the captured rounds did not suffice.
+    // Continue host-polled, rounds_per_chunk rounds at a time, when the
+    // captured rounds did not suffice (push schedules vary by a round or two).
+    const std::uint32_t chunk = std::max(1u, o.rounds_per_chunk);
     while (!hc->done) {
-      for (std::uint32_t r = 0; r < rounds; ++r)
+      for (std::uint32_t r = 0; r < chunk; ++r)
         BFN_HIP(relax_launch(ws->block, false, false, ws->grid, ws->lds_bytes, ws->max_v,
                              ws->work_words, g->view, b, s));
-      ws->launches += 2ull * rounds;
+      ws->launches += 2ull * chunk;
       BFN_HIP(hipMemcpyAsync(ws->h_control.ptr, ws->control.ptr, sizeof(DevControl),
                              hipMemcpyDeviceToHost, s));
       BFN_HIP(hipStreamSynchronize(s));

@@ -1,0 +1,355 @@
+// api_test.cpp -- tests of the reference-named C++ API (gpu_api records and
+// rules, batch descriptions, resident graph, batched and single-query
+// engines) over the C ABI.
+//
+//   api_test cpu           rules and host layouts only; no device call
+//   api_test gpu OUT.bin   runs every engine on the 16x16x64 synthetic graph
+//                          (seed 1), checks the engines against each other,
+//                          and writes the lanes and labels to OUT.bin for the
+//                          oracle comparison done by tests/test_cpp_api.py
+//
+// The reference states the same rules in tests/gpu_api_test.cpp and its
+// batched engine tests; the cases below are this library's own.
+#include "bfnew/batch_layout.hpp"
+#include "bfnew/gpu_api.hpp"
+#include "bfnew/hip/engines.hpp"
+#include "bfnew/hip/runtime.hpp"
+#include "bfnew/query.hpp"
+#include "bfnew/synthetic.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+using namespace bfnew;
+
+namespace {
+
+int failures = 0;
+
+void check(bool ok, const char* what, int line) {
+  if (!ok) {
+    std::fprintf(stderr, "FAIL line %d: %s\n", line, what);
+    ++failures;
+  }
+}
+#define CHECK(x) check((x), #x, __LINE__)
+
+template <typename E, typename F>
+bool throws(F&& f) {
+  try {
+    f();
+  } catch (const E&) {
+    return true;
+  } catch (...) {
+    return false;
+  }
+  return false;
+}
+
+GpuRunOptions options_for(EngineKind kind) {
+  GpuRunOptions o;
+  o.engine = kind;
+  o.control_mode = ControlMode::persistent_cooperative;
+  o.maximum_rounds = 1U << 20;
+  return o;
+}
+
+// ---------------------------------------------------------------------------
+void cpu_rules() {
+  // Run options.
+  GpuRunOptions o;
+  CHECK(validate_gpu_run_options(o) == GpuRunOptionsError::none);
+  GpuRunOptions bad = o;
+  bad.engine = static_cast<EngineKind>(7);
+  CHECK(validate_gpu_run_options(bad) == GpuRunOptionsError::invalid_engine);
+  bad = o;
+  bad.rounds_per_chunk = 0;
+  CHECK(validate_gpu_run_options(bad) == GpuRunOptionsError::invalid_rounds_per_chunk);
+  bad = o;
+  bad.grid_policy = GridPolicy::fixed_blocks_per_wgp;
+  CHECK(validate_gpu_run_options(bad) == GpuRunOptionsError::invalid_blocks_per_wgp);
+  bad.blocks_per_wgp = 2;
+  CHECK(validate_gpu_run_options(bad) == GpuRunOptionsError::none);
+  bad = o;
+  bad.blocks_per_wgp = 2;  // a derived grid takes no count
+  CHECK(validate_gpu_run_options(bad) == GpuRunOptionsError::invalid_blocks_per_wgp);
+  bad = o;
+  bad.maximum_rounds = 0;
+  CHECK(validate_gpu_run_options(bad) == GpuRunOptionsError::invalid_maximum_rounds);
+  bad = o;
+  bad.enable_per_lane_convergence = 2;
+  CHECK(validate_gpu_run_options(bad) == GpuRunOptionsError::invalid_per_lane_flag);
+
+  // Controllers: initial states are valid; each broken invariant is named.
+  for (EngineKind k : {EngineKind::jacobi_pull, EngineKind::dense_chaotic_push,
+                       EngineKind::frontier_push}) {
+    GpuRunOptions ko = o;
+    ko.engine = k;
+    const DeviceController c = initialize_device_controller(ko, 0x5U, 3U);
+    CHECK(validate_device_controller(c) == DeviceControllerError::none);
+    CHECK(c.distance_write_slot == (k == EngineKind::jacobi_pull ? 1U : 0U));
+  }
+  GpuRunOptions fo = o;
+  fo.engine = EngineKind::frontier_push;
+  CHECK(throws<std::invalid_argument>([&] { (void)initialize_device_controller(fo, 1U, 0U); }));
+  CHECK(throws<std::invalid_argument>([&] { (void)initialize_device_controller(o, 0U); }));
+  DeviceController c = initialize_device_controller(o, 0x3U);
+  DeviceController d = c;
+  d.active_lane_mask = 0x7U;
+  CHECK(validate_device_controller(d) == DeviceControllerError::mask_outside_valid_lanes);
+  d = c;
+  d.converged_lane_mask = 0x1U;
+  CHECK(validate_device_controller(d) == DeviceControllerError::active_converged_overlap);
+  d = c;
+  d.distance_write_slot = 0U;
+  CHECK(validate_device_controller(d) == DeviceControllerError::invalid_distance_slots);
+  d = c;  // a converged stop: no active lane, every lane converged, no error
+  d.done = 1U;
+  d.execute_lane_mask = 0U;
+  d.active_lane_mask = 0U;
+  d.converged_lane_mask = 0x3U;
+  d.stop_reason = static_cast<std::uint32_t>(DeviceStopReason::converged);
+  CHECK(validate_device_controller(d) == DeviceControllerError::none);
+  d.error_bits = device_error::queue_overflow;
+  CHECK(validate_device_controller(d) == DeviceControllerError::inconsistent_error_state);
+  d.error_bits = 0x80U;
+  CHECK(validate_device_controller(d) == DeviceControllerError::unknown_error_bits);
+  d = c;
+  d.done = 1U;
+  d.execute_lane_mask = 0U;
+  d.stop_reason = static_cast<std::uint32_t>(DeviceStopReason::maximum_rounds);
+  CHECK(validate_device_controller(d) == DeviceControllerError::inconsistent_terminal_state);
+  d.rounds_completed = d.maximum_rounds;
+  CHECK(validate_device_controller(d) == DeviceControllerError::none);
+
+  // Run status.
+  DeviceRunStatus s;
+  s.valid_lane_mask = 0x3U;
+  s.converged = 1U;
+  s.converged_lane_mask = 0x3U;
+  s.reached_target_mask = 0x1U;
+  s.bounding_box_miss_mask = 0x2U;
+  s.stop_reason = static_cast<std::uint32_t>(DeviceStopReason::converged);
+  CHECK(validate_device_run_status(s) == DeviceRunStatusError::none);
+  DeviceRunStatus t = s;
+  t.reached_target_mask = 0x3U;
+  CHECK(validate_device_run_status(t) == DeviceRunStatusError::reached_miss_overlap);
+  t = s;
+  t.converged = 0U;
+  CHECK(validate_device_run_status(t) == DeviceRunStatusError::inconsistent_convergence_state);
+  t = s;
+  t.converged = 0U;
+  t.stop_reason = static_cast<std::uint32_t>(DeviceStopReason::device_failure);
+  t.error_bits = device_error::device_failure;
+  CHECK(validate_device_run_status(t) == DeviceRunStatusError::error_reported_as_bounding_box_miss);
+}
+
+void cpu_layouts() {
+  SyntheticTileGridConfig cfg;
+  cfg.tiles_x = 8;
+  cfg.tiles_y = 8;
+  cfg.nodes_per_tile = 16;
+  cfg.seed = 3;
+  const SyntheticTileGraph gen = SyntheticTileGraph::generate(cfg, 1);
+  const WeightedGraph g = gen.weighted_graph();
+  const hip::ResidentGraphPlan plan = hip::make_resident_graph_plan(gen.layout());
+  const auto offs = gen.tile_vertex_offsets();
+  CHECK(plan.tile_vertex_offsets.size() == offs.size());
+  CHECK(std::equal(offs.begin(), offs.end(), plan.tile_vertex_offsets.begin()));
+  CHECK(plan.fingerprint == fingerprint_device_graph_layout32(gen.layout()));
+  DeviceGraphLayout32 scrambled = gen.layout();
+  std::swap(scrambled.owner_tiles.front(), scrambled.owner_tiles.back());
+  CHECK(throws<std::invalid_argument>([&] { (void)hip::make_resident_graph_plan(scrambled); }));
+
+  std::vector<RouteQuery> qs;
+  const std::vector<VertexId> s0{VertexId{5}}, t0{VertexId{200}};
+  const std::vector<VertexId> s1{VertexId{700}, VertexId{3}}, t1{VertexId{900}};
+  qs.push_back(make_route_query(QueryId{10}, g, s0, t0, 1));
+  qs.push_back(make_route_query(QueryId{11}, g, s1, t1, 0));
+  BatchDeviceDescription d;
+  prepare_batch_device_description(g, qs, 4, d);
+  CHECK(validate_batch_device_description(g, d).ok());
+  CHECK(d.valid_lane_mask == 0x3U);
+  CHECK(d.source_offsets == (std::vector<std::uint32_t>{0, 1, 3, 3, 3}));
+  CHECK(d.sources == (std::vector<std::uint32_t>{5, 3, 700}));
+  std::size_t lane0 = 0;
+  for (std::uint32_t t : d.union_tiles) lane0 += d.tile_lane_masks[t] & 1U;
+  CHECK(lane0 == qs[0].selected_tiles.size());
+  BatchDeviceDescription e = d;
+  e.union_tiles.pop_back();
+  CHECK(validate_batch_device_description(g, e).code ==
+        BatchLayoutValidationErrorCode::invalid_union_tiles);
+  e = d;
+  std::swap(e.sources[1], e.sources[2]);  // lane 1 sources out of order
+  CHECK(validate_batch_device_description(g, e).code ==
+        BatchLayoutValidationErrorCode::invalid_terminal_payload);
+  e = d;
+  e.tile_lane_masks[d.union_tiles[0]] |= 0x8U;  // a bit of an invalid lane
+  CHECK(validate_batch_device_description(g, e).code ==
+        BatchLayoutValidationErrorCode::invalid_tile_lane_masks);
+  CHECK(throws<std::invalid_argument>([&] { prepare_batch_device_description(g, qs, 1, d); }));
+}
+
+// ---------------------------------------------------------------------------
+struct Writer {
+  std::FILE* f;
+  void u32(std::uint32_t v) { std::fwrite(&v, 4, 1, f); }
+  void f32s(const std::vector<float>& v) { std::fwrite(v.data(), 4, v.size(), f); }
+};
+
+int gpu(const char* out_path) {
+  SyntheticTileGridConfig cfg;  // 16 x 16 tiles x 64 nodes, seed 1
+  const SyntheticTileGraph gen = SyntheticTileGraph::generate(cfg, 0);
+  const WeightedGraph g = gen.weighted_graph();
+  const TileRunLayout64 runs = build_tile_run_layout(g);
+  hip::HipStream stream;
+  hip::ResidentDeviceGraph resident;
+  resident.upload_once_async(hip::make_resident_graph_plan(gen.layout()), stream);
+  resident.wait_until_ready(stream);
+  CHECK(throws<std::logic_error>(
+      [&] { resident.upload_once_async(hip::make_resident_graph_plan(gen.layout()), stream); }));
+
+  SyntheticQueryConfig qc;
+  qc.count = 6;
+  qc.seed = 17;
+  qc.side_min = 3;
+  qc.side_max = 10;
+  std::vector<RouteQuery> qs;
+  for (const SyntheticQuery& sq : generate_synthetic_queries(cfg, qc)) {
+    const std::vector<VertexId> s{VertexId{sq.source}}, t{VertexId{sq.target}};
+    qs.push_back(make_route_query(QueryId{sq.query_id}, g, s, t, 2));
+  }
+  const std::uint32_t W = 8;  // six valid lanes, two padded
+  BatchDeviceDescription batch;
+  prepare_batch_device_description(g, qs, W, batch);
+  const std::uint32_t V = g.vertex_count();
+
+  hip::ReusableBatchedWorkspace ws(stream);
+  hip::BatchedJacobiPullEngine jacobi(g, runs, resident, ws, stream);
+  hip::BatchedDenseChaoticPushEngine dense(g, runs, resident, ws, stream);
+  hip::BatchedFrontierPushEngine frontier(g, runs, resident, ws, stream);
+
+  const hip::BatchedRunOutput ref = jacobi.run_with_distances(batch, options_for(EngineKind::jacobi_pull));
+  CHECK(ref.result.status.converged == 1U);
+  CHECK(ref.result.status.valid_lane_mask == 0x3FU);
+  CHECK(validate_device_run_status(ref.result.status) == DeviceRunStatusError::none);
+  CHECK(ref.distances.size() == static_cast<std::size_t>(V) * W);
+  for (std::uint32_t l = 0; l < 6; ++l) CHECK(ref.lane_convergence_rounds[l] > 0U);
+  CHECK(ref.lane_convergence_rounds[6] == 0U && ref.lane_convergence_rounds[7] == 0U);
+
+  auto same_bits = [](const std::vector<float>& a, const std::vector<float>& b) {
+    return a.size() == b.size() && std::memcmp(a.data(), b.data(), 4 * a.size()) == 0;
+  };
+  const auto d_out = dense.run_with_distances(batch, options_for(EngineKind::dense_chaotic_push));
+  const auto f_out = frontier.run_with_distances(batch, options_for(EngineKind::frontier_push));
+  CHECK(same_bits(d_out.distances, ref.distances));
+  CHECK(same_bits(f_out.distances, ref.distances));
+  GpuRunOptions chunked = options_for(EngineKind::jacobi_pull);
+  chunked.control_mode = ControlMode::chunked_host_poll;
+  chunked.instrumentation = InstrumentationLevel::light;
+  const auto j2 = jacobi.run_with_distances(batch, chunked);
+  CHECK(same_bits(j2.distances, ref.distances));
+  CHECK(j2.result.status.rounds_completed == ref.result.status.rounds_completed);
+  CHECK(j2.metrics.tile_visits > 0U);
+  const DeviceRunStatus compact = frontier.run_compact_status(batch, options_for(EngineKind::frontier_push));
+  CHECK(compact.converged == 1U && compact.valid_lane_mask == 0x3FU);
+  CHECK(compact.reached_target_mask == ref.result.status.reached_target_mask);
+
+  // Wrong engine for the class, and a round limit that stops early.
+  CHECK(throws<std::invalid_argument>(
+      [&] { (void)jacobi.run_status_only(batch, options_for(EngineKind::frontier_push)); }));
+  GpuRunOptions one_round = options_for(EngineKind::jacobi_pull);
+  one_round.maximum_rounds = 1U;
+  const auto early = jacobi.run_status_only(batch, one_round);
+  CHECK(early.result.status.converged == 0U);
+  CHECK(early.result.status.stop_reason == static_cast<std::uint32_t>(DeviceStopReason::maximum_rounds));
+  CHECK(validate_device_run_status(early.result.status) == DeviceRunStatusError::none);
+
+  hip::ReusableCompactPathWorkspace pws;
+  const hip::CompactPathBatchOutput paths =
+      dense.run_compact_paths(batch, options_for(EngineKind::dense_chaotic_push), pws);
+  CHECK(paths.targets.size() == batch.targets.size());
+
+  // Single-query engines reproduce each lane.
+  hip::ReusableDeviceWorkspace single_ws(stream);
+  hip::JacobiPullEngine sj(g, runs, resident, single_ws, stream);
+  hip::DenseChaoticPushEngine sd(g, runs, resident, single_ws, stream);
+  hip::FrontierPushEngine sf(g, runs, resident, single_ws, stream);
+  GpuSsspEngine* singles[] = {&sj, &sd, &sf};
+  for (std::uint32_t l = 0; l < qs.size(); ++l) {
+    std::vector<float> lane(V);
+    for (std::uint32_t v = 0; v < V; ++v) lane[v] = ref.distances[static_cast<std::size_t>(v) * W + l];
+    for (GpuSsspEngine* e : singles) {
+      auto* base = static_cast<hip::SingleQueryEngineBase*>(e);
+      const auto out = base->run_with_distances(qs[l], options_for(e->kind()));
+      CHECK(out.result.status.converged == 1U);
+      CHECK(same_bits(out.distances, lane));
+      const GpuSsspResult r = e->run(qs[l], options_for(e->kind()));
+      CHECK(r.status.converged == 1U && r.engine_kind == static_cast<std::uint32_t>(e->kind()));
+    }
+    const auto sel = sj.run_with_selected_distances(qs[l], options_for(EngineKind::jacobi_pull));
+    std::size_t expect = 0;
+    for (TileId t : qs[l].selected_tiles)
+      expect += g.tile_vertex_offsets()[t.value() + 1] - g.tile_vertex_offsets()[t.value()];
+    CHECK(sel.distances.size() == expect);
+  }
+
+  // Lanes, labels and paths for the oracle comparison.
+  std::FILE* f = std::fopen(out_path, "wb");
+  if (!f) {
+    std::fprintf(stderr, "cannot write %s\n", out_path);
+    return 2;
+  }
+  Writer w{f};
+  w.u32(V);
+  w.u32(W);
+  w.u32(static_cast<std::uint32_t>(qs.size()));
+  for (const RouteQuery& q : qs) {
+    w.u32(q.sources[0].value());
+    w.u32(q.targets[0].value());
+    w.u32(static_cast<std::uint32_t>(q.selected_tiles.size()));
+    for (TileId t : q.selected_tiles) w.u32(t.value());
+  }
+  w.f32s(ref.distances);
+  for (const CompactTargetPath& p : paths.targets) {
+    w.u32(static_cast<std::uint32_t>(p.summary.reconstruction));
+    w.u32(static_cast<std::uint32_t>(p.vertices.size()));
+    for (VertexId v : p.vertices) w.u32(v.value());
+    for (EdgeId e : p.edge_ids) w.u32(static_cast<std::uint32_t>(e.value()));
+    w.f32s(p.distance_labels);
+  }
+  std::fclose(f);
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 2) {
+    std::fprintf(stderr, "usage: api_test cpu | gpu OUT.bin\n");
+    return 2;
+  }
+  int rc = 0;
+  try {
+    if (std::strcmp(argv[1], "cpu") == 0) {
+      cpu_rules();
+      cpu_layouts();
+    } else if (std::strcmp(argv[1], "gpu") == 0 && argc >= 3) {
+      rc = gpu(argv[2]);
+    } else {
+      std::fprintf(stderr, "unknown mode\n");
+      return 2;
+    }
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "exception: %s\n", e.what());
+    return 3;
+  }
+  std::printf("%s: %d failure(s)\n", argv[1], failures);
+  return failures ? 1 : rc;
+}
