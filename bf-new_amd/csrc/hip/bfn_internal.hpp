@@ -15,6 +15,10 @@
 //                       (k indexes the destination tile's in-neighbour list)
 //                       csr_pack[E] = k << 24 | src_local << 12 | dst_local
 //                       (k indexes the source tile's out-neighbour list)
+//     relax streams   : csc_rec[E] / csr_rec[E] = (pack, weight bits), each
+//                       tile's block reordered [internal | boundary]; the
+//                       engines keep the internal part in registers for the
+//                       tile closure and stream the boundary part once
 //
 //   batch (one workspace; any number of queries)
 //     slot s = (query q, selected tile T); slots of q are q's selected tiles in
@@ -77,6 +81,12 @@ struct DevGraph {
   const std::uint32_t* out_nbr_tiles;
   const std::uint32_t* csc_pack;
   const std::uint32_t* csr_pack;
+  // Relaxation streams: each tile's block as [internal | boundary] records
+  // (pack, weight bits); *_intra[T] = internal records of tile T.
+  const uint2* csc_rec;
+  const uint2* csr_rec;
+  const std::uint32_t* csc_intra;
+  const std::uint32_t* csr_intra;
 };
 
 // Control block; lives in device memory, one per workspace.

@@ -27,19 +27,43 @@ hipError_t relax_occupancy(std::uint32_t block, bool persistent, std::size_t lds
                            int* blocks_per_cu);
 hipError_t prep_launch(const DevGraph& g, const DevBatch& b, std::uint64_t pool_words,
                        hipStream_t stream);
-hipError_t derive_tile_image(std::uint32_t tile_count, const std::uint32_t* tile_vertex_offsets,
-                             const std::uint32_t* owner_tiles, const std::uint32_t* row_offsets,
-                             const std::uint32_t* destinations, const std::uint32_t* row_runs,
-                             const std::uint32_t* run_dst_tiles, const float* weights,
-                             const std::uint32_t* column_offsets, const std::uint32_t* sources,
-                             const std::uint32_t* col_runs, const std::uint32_t* run_src_tiles,
-                             const float* in_weights, std::uint32_t* counts_scratch,
-                             std::uint32_t* in_offsets, std::uint32_t* out_offsets,
-                             std::uint32_t** in_tiles, std::uint32_t** in_stage,
-                             std::uint32_t** out_tiles, std::uint32_t* csc_pack,
-                             std::uint32_t* csr_pack, std::uint32_t* stats,
-                             std::uint32_t* host_in_total, std::uint32_t* host_out_total,
-                             hipStream_t stream);
+// Inputs and outputs of the tile-image derivation (graph.hip).
+struct TileImageArgs {
+  std::uint32_t tile_count;
+  const std::uint32_t* tile_vertex_offsets;
+  const std::uint32_t* owner_tiles;
+  const std::uint32_t* row_offsets;
+  const std::uint32_t* destinations;
+  const std::uint32_t* row_runs;
+  const std::uint32_t* run_dst_tiles;
+  const float* weights;
+  const std::uint32_t* column_offsets;
+  const std::uint32_t* sources;
+  const std::uint32_t* col_runs;
+  const std::uint32_t* run_src_tiles;
+  const float* in_weights;
+  std::uint32_t* counts_scratch;
+  std::uint32_t* in_offsets;
+  std::uint32_t* out_offsets;
+  std::uint32_t** in_tiles;   // allocated here
+  std::uint32_t** in_stage;   // allocated here
+  std::uint32_t** out_tiles;  // allocated here
+  std::uint32_t* csc_pack;
+  std::uint32_t* csr_pack;
+  uint2* csc_rec;
+  uint2* csr_rec;
+  std::uint32_t* csc_perm;
+  std::uint32_t* csr_perm;
+  std::uint32_t* csc_intra;
+  std::uint32_t* csr_intra;
+  std::uint32_t* stats;
+  std::uint32_t* host_in_total;
+  std::uint32_t* host_out_total;
+};
+hipError_t derive_tile_image(const TileImageArgs& args, hipStream_t stream);
+// Re-reads the record weights from the orientation weight arrays.
+hipError_t refresh_record_weights(const DevGraph& g, const std::uint32_t* csr_perm,
+                                  const std::uint32_t* csc_perm, hipStream_t stream);
 hipError_t paths_launch(const PathArgs& a, std::uint32_t query_count, hipStream_t stream);
 hipError_t scan_u64_launch(const unsigned long long* in, unsigned long long* out, std::uint32_t n,
                            hipStream_t stream);

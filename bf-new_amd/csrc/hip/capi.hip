@@ -116,8 +116,8 @@ struct bfn_graph_s {
   DevGraph view{};
   std::uint32_t V = 0, E = 0, T = 0, Rcsr = 0, Rcsc = 0;
   DeviceBuffer owner, tile_off, row_off, dst, w, row_runs, run_edges, run_dst, col_off, src, cw,
-      eid, col_runs, crun_edges, crun_src, in_off, out_off, csc_pack, csr_pack, base_cost,
-      occ_scratch, counts, stats;
+      eid, col_runs, crun_edges, crun_src, in_off, out_off, csc_pack, csr_pack, csc_rec, csr_rec,
+      csc_perm, csr_perm, csc_intra, csr_intra, base_cost, occ_scratch, counts, stats;
   std::uint32_t* in_tiles = nullptr;
   std::uint32_t* in_stage = nullptr;
   std::uint32_t* out_tiles = nullptr;
@@ -316,24 +316,52 @@ int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_gra
   if (e == hipSuccess) e = g->in_off.reserve(sizeof(std::uint32_t) * (T + 1));
   if (e == hipSuccess) e = g->out_off.reserve(sizeof(std::uint32_t) * (T + 1));
   if (e == hipSuccess) e = g->counts.reserve(sizeof(std::uint32_t) * (T + 1));
-  if (e == hipSuccess) e = g->csc_pack.reserve(sizeof(std::uint32_t) * std::max(E, 1u));
-  if (e == hipSuccess) e = g->csr_pack.reserve(sizeof(std::uint32_t) * std::max(E, 1u));
+  const std::size_t e_words = std::max(E, 1u);
+  if (e == hipSuccess) e = g->csc_pack.reserve(sizeof(std::uint32_t) * e_words);
+  if (e == hipSuccess) e = g->csr_pack.reserve(sizeof(std::uint32_t) * e_words);
+  if (e == hipSuccess) e = g->csc_rec.reserve(sizeof(uint2) * e_words);
+  if (e == hipSuccess) e = g->csr_rec.reserve(sizeof(uint2) * e_words);
+  if (e == hipSuccess) e = g->csc_perm.reserve(sizeof(std::uint32_t) * e_words);
+  if (e == hipSuccess) e = g->csr_perm.reserve(sizeof(std::uint32_t) * e_words);
+  if (e == hipSuccess) e = g->csc_intra.reserve(sizeof(std::uint32_t) * (T + 1));
+  if (e == hipSuccess) e = g->csr_intra.reserve(sizeof(std::uint32_t) * (T + 1));
   if (e == hipSuccess) e = g->stats.reserve(sizeof(std::uint32_t) * 8);
   if (e == hipSuccess) e = hipMemsetAsync(g->stats.ptr, 0, sizeof(std::uint32_t) * 8, s);
   if (e != hipSuccess) return cleanup(BFN_E_HIP, std::string("graph upload: ") + hipGetErrorString(e));
 
   std::uint32_t in_total = 0, out_total = 0;
-  e = derive_tile_image(T, g->tile_off.as<std::uint32_t>(), g->owner.as<std::uint32_t>(),
-                        g->row_off.as<std::uint32_t>(), g->dst.as<std::uint32_t>(),
-                        g->row_runs.as<std::uint32_t>(), g->run_dst.as<std::uint32_t>(),
-                        g->w.as<float>(), g->col_off.as<std::uint32_t>(),
-                        g->src.as<std::uint32_t>(), g->col_runs.as<std::uint32_t>(),
-                        g->crun_src.as<std::uint32_t>(), g->cw.as<float>(),
-                        g->counts.as<std::uint32_t>(), g->in_off.as<std::uint32_t>(),
-                        g->out_off.as<std::uint32_t>(), &g->in_tiles, &g->in_stage,
-                        &g->out_tiles, g->csc_pack.as<std::uint32_t>(),
-                        g->csr_pack.as<std::uint32_t>(), g->stats.as<std::uint32_t>(), &in_total,
-                        &out_total, s);
+  TileImageArgs ta{};
+  ta.tile_count = T;
+  ta.tile_vertex_offsets = g->tile_off.as<std::uint32_t>();
+  ta.owner_tiles = g->owner.as<std::uint32_t>();
+  ta.row_offsets = g->row_off.as<std::uint32_t>();
+  ta.destinations = g->dst.as<std::uint32_t>();
+  ta.row_runs = g->row_runs.as<std::uint32_t>();
+  ta.run_dst_tiles = g->run_dst.as<std::uint32_t>();
+  ta.weights = g->w.as<float>();
+  ta.column_offsets = g->col_off.as<std::uint32_t>();
+  ta.sources = g->src.as<std::uint32_t>();
+  ta.col_runs = g->col_runs.as<std::uint32_t>();
+  ta.run_src_tiles = g->crun_src.as<std::uint32_t>();
+  ta.in_weights = g->cw.as<float>();
+  ta.counts_scratch = g->counts.as<std::uint32_t>();
+  ta.in_offsets = g->in_off.as<std::uint32_t>();
+  ta.out_offsets = g->out_off.as<std::uint32_t>();
+  ta.in_tiles = &g->in_tiles;
+  ta.in_stage = &g->in_stage;
+  ta.out_tiles = &g->out_tiles;
+  ta.csc_pack = g->csc_pack.as<std::uint32_t>();
+  ta.csr_pack = g->csr_pack.as<std::uint32_t>();
+  ta.csc_rec = g->csc_rec.as<uint2>();
+  ta.csr_rec = g->csr_rec.as<uint2>();
+  ta.csc_perm = g->csc_perm.as<std::uint32_t>();
+  ta.csr_perm = g->csr_perm.as<std::uint32_t>();
+  ta.csc_intra = g->csc_intra.as<std::uint32_t>();
+  ta.csr_intra = g->csr_intra.as<std::uint32_t>();
+  ta.stats = g->stats.as<std::uint32_t>();
+  ta.host_in_total = &in_total;
+  ta.host_out_total = &out_total;
+  e = derive_tile_image(ta, s);
   if (e != hipSuccess) return cleanup(BFN_E_HIP, std::string("tile image: ") + hipGetErrorString(e));
   std::uint32_t stats[8] = {};
   g->h_tile_off.assign(L->tile_vertex_offsets, L->tile_vertex_offsets + T + 1);
@@ -374,6 +402,10 @@ int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_gra
   v.out_nbr_tiles = g->out_tiles;
   v.csc_pack = g->csc_pack.as<std::uint32_t>();
   v.csr_pack = g->csr_pack.as<std::uint32_t>();
+  v.csc_rec = g->csc_rec.as<uint2>();
+  v.csr_rec = g->csr_rec.as<uint2>();
+  v.csc_intra = g->csc_intra.as<std::uint32_t>();
+  v.csr_intra = g->csr_intra.as<std::uint32_t>();
 
   bfn_graph_info& info = g->info;
   info.max_tile_vertices = max_v;
@@ -387,8 +419,11 @@ int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_gra
   }
   info.max_stage_floats = stats[1];
   info.has_zero_weight = stats[4];
+  // reference image + derived tile lists + packs (4 B) + records (8 B) +
+  // record permutations (4 B) per edge and orientation
   info.device_bytes = 4ull * (2ull * V + 2 + T + 1 + 6ull * E + 2ull * (g->Rcsr + g->Rcsc + 2) +
-                              2ull * (T + 1) + 2ull * in_total + out_total);
+                              4ull * (T + 1) + 2ull * in_total + out_total) +
+                      2ull * E * (4 + 8 + 4);
   *out = g;
   return BFN_OK;
 }
@@ -420,6 +455,8 @@ int32_t bfn_graph_update_weights(bfn_graph graph, const float* csr_weights,
                          hipMemcpyHostToDevice, graph->stream));
   BFN_HIP(hipMemcpyAsync(graph->cw.ptr, csc_weights, sizeof(float) * graph->E,
                          hipMemcpyHostToDevice, graph->stream));
+  BFN_HIP(refresh_record_weights(graph->view, graph->csr_perm.as<std::uint32_t>(),
+                                 graph->csc_perm.as<std::uint32_t>(), graph->stream));
   BFN_HIP(hipStreamSynchronize(graph->stream));
   return BFN_OK;
 }
@@ -438,6 +475,8 @@ int32_t apply_node_costs(bfn_graph graph, const float* base_cost, const int* dev
   }
   BFN_HIP(node_cost_launch(graph->view, graph->base_cost.as<float>(), device_occ,
                            graph->w.as<float>(), graph->cw.as<float>(), graph->stream));
+  BFN_HIP(refresh_record_weights(graph->view, graph->csr_perm.as<std::uint32_t>(),
+                                 graph->csc_perm.as<std::uint32_t>(), graph->stream));
   BFN_HIP(hipStreamSynchronize(graph->stream));
   return BFN_OK;
 }

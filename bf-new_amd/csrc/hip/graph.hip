@@ -7,11 +7,18 @@
 //   out_nbr_tiles -- T, then the other destination tiles of T's CSR runs
 //   in_nbr_stage  -- LDS float offset of each in-neighbour slab (prefix of
 //                    tile sizes in list order, so T's own slab is at 0)
-//   csc_pack / csr_pack -- one u32 per edge: neighbour index << 24 |
-//                    source-local id << 12 | destination-local id
+//   csc_pack / csr_pack -- one u32 per edge at its orientation position:
+//                    neighbour index << 24 | source-local id << 12 |
+//                    destination-local id
+//   csc_rec / csr_rec -- the relaxation stream: per tile, the tile's edge
+//                    block reordered as [internal edges | boundary edges]
+//                    (each part in orientation order), one (pack, weight
+//                    bits) record per edge; *_perm[p] is the orientation
+//                    position of record p and *_intra[T] the internal count.
 // One workgroup per tile; the distinct tiles of the tile's runs are collected
 // in an LDS hash set, ranked in LDS, and the packs are written per vertex.
 #include "bfn_internal.hpp"
+#include "bfn_launch.hpp"
 
 namespace bfn {
 
@@ -150,6 +157,66 @@ __global__ void fill_pack_kernel(DeriveArgs a) {
   }
 }
 
+struct SplitArgs {
+  const std::uint32_t* tile_vertex_offsets;
+  const std::uint32_t* offsets;  // [V+1] orientation offsets
+  const std::uint32_t* pack;     // [E] at orientation positions
+  const float* weights;          // [E] at orientation positions
+  uint2* rec;
+  std::uint32_t* perm;
+  std::uint32_t* intra;          // [T]
+};
+
+// Stable partition of each tile's block into internal and boundary records.
+// A 64-lane ballot ranks the internal edges of a chunk inside each wave, the
+// wave totals are scanned in LDS, and a carry runs across chunks.
+__global__ __launch_bounds__(256) void split_records_kernel(SplitArgs a) {
+  __shared__ std::uint32_t wave_total[16];
+  __shared__ std::uint32_t internal_total;
+  const std::uint32_t T = blockIdx.x;
+  const std::uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const std::uint32_t e0 = a.offsets[a.tile_vertex_offsets[T]];
+  const std::uint32_t e1 = a.offsets[a.tile_vertex_offsets[T + 1]];
+  if (tid == 0) internal_total = 0;
+  __syncthreads();
+  std::uint32_t mine = 0;
+  for (std::uint32_t e = e0 + tid; e < e1; e += blockDim.x) mine += (a.pack[e] >> 24) == 0u;
+  if (mine) atomicAdd(&internal_total, mine);
+  __syncthreads();
+  const std::uint32_t n_int = internal_total;
+  std::uint32_t carry = 0;  // internal edges before the current chunk
+  for (std::uint32_t base = e0; base < e1; base += blockDim.x) {
+    const std::uint32_t e = base + tid;
+    const bool in_range = e < e1;
+    const std::uint32_t p = in_range ? a.pack[e] : kNone;
+    const bool internal = in_range && (p >> 24) == 0u;
+    const unsigned long long m = __ballot(internal);
+    const std::uint32_t below = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_total[wave] = __popcll(m);
+    __syncthreads();
+    std::uint32_t before = carry;
+    for (std::uint32_t w = 0; w < wave; ++w) before += wave_total[w];
+    std::uint32_t chunk = 0;
+    for (std::uint32_t w = 0; w < (blockDim.x >> 6); ++w) chunk += wave_total[w];
+    if (in_range) {
+      const std::uint32_t rank_int = before + below;               // internal edges before e
+      const std::uint32_t rank_bnd = (e - e0) - rank_int;          // boundary edges before e
+      const std::uint32_t dst = internal ? e0 + rank_int : e0 + n_int + rank_bnd;
+      a.rec[dst] = make_uint2(p, __float_as_uint(a.weights[e]));
+      a.perm[dst] = e;
+    }
+    carry += chunk;
+    __syncthreads();
+  }
+  if (tid == 0) a.intra[T] = n_int;
+}
+
+__global__ void refresh_weights_kernel(uint2* rec, const std::uint32_t* perm, const float* w,
+                                       std::uint32_t n) {
+  const std::uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) rec[p].y = __float_as_uint(w[perm[p]]);
+}
+
 __global__ void exclusive_scan_kernel(const std::uint32_t* in, std::uint32_t* out,
                                       std::uint32_t n) {
   // Single workgroup: chunked Hillis-Steele in LDS, carry across chunks.
@@ -180,50 +247,60 @@ __global__ void exclusive_scan_kernel(const std::uint32_t* in, std::uint32_t* ou
 
 // Runs both orientations.  stats[0] != 0 means a tile has more than 255
 // neighbour tiles (not representable in the 8-bit neighbour index).
-hipError_t derive_tile_image(std::uint32_t tile_count, const std::uint32_t* tile_vertex_offsets,
-                             const std::uint32_t* owner_tiles, const std::uint32_t* row_offsets,
-                             const std::uint32_t* destinations, const std::uint32_t* row_runs,
-                             const std::uint32_t* run_dst_tiles, const float* weights,
-                             const std::uint32_t* column_offsets, const std::uint32_t* sources,
-                             const std::uint32_t* col_runs, const std::uint32_t* run_src_tiles,
-                             const float* in_weights, std::uint32_t* counts_scratch,
-                             std::uint32_t* in_offsets, std::uint32_t* out_offsets,
-                             std::uint32_t** in_tiles, std::uint32_t** in_stage,
-                             std::uint32_t** out_tiles, std::uint32_t* csc_pack,
-                             std::uint32_t* csr_pack, std::uint32_t* stats,
-                             std::uint32_t* host_in_total, std::uint32_t* host_out_total,
-                             hipStream_t stream) {
-  DeriveArgs in{tile_count, tile_vertex_offsets, owner_tiles, column_offsets, sources, col_runs,
-                run_src_tiles, counts_scratch, in_offsets, nullptr, nullptr, csc_pack, stats,
-                in_weights, true};
-  DeriveArgs out{tile_count, tile_vertex_offsets, owner_tiles, row_offsets, destinations, row_runs,
-                 run_dst_tiles, counts_scratch, out_offsets, nullptr, nullptr, csr_pack, stats,
-                 weights, false};
+hipError_t derive_tile_image(const TileImageArgs& t, hipStream_t stream) {
+  DeriveArgs in{t.tile_count, t.tile_vertex_offsets, t.owner_tiles, t.column_offsets, t.sources,
+                t.col_runs, t.run_src_tiles, t.counts_scratch, t.in_offsets, nullptr, nullptr,
+                t.csc_pack, t.stats, t.in_weights, true};
+  DeriveArgs out{t.tile_count, t.tile_vertex_offsets, t.owner_tiles, t.row_offsets,
+                 t.destinations, t.row_runs, t.run_dst_tiles, t.counts_scratch, t.out_offsets,
+                 nullptr, nullptr, t.csr_pack, t.stats, t.weights, false};
   hipError_t e;
   // In-neighbour lists.
-  hipLaunchKernelGGL(count_kernel, dim3(tile_count), dim3(256), 0, stream, in);
-  hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(1024), 0, stream, counts_scratch,
-                     in_offsets, tile_count);
-  if ((e = hipMemcpyAsync(host_in_total, in_offsets + tile_count, sizeof(std::uint32_t),
+  hipLaunchKernelGGL(count_kernel, dim3(t.tile_count), dim3(256), 0, stream, in);
+  hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(1024), 0, stream, t.counts_scratch,
+                     t.in_offsets, t.tile_count);
+  if ((e = hipMemcpyAsync(t.host_in_total, t.in_offsets + t.tile_count, sizeof(std::uint32_t),
                           hipMemcpyDeviceToHost, stream)) != hipSuccess)
     return e;
   if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
-  if ((e = hipMalloc(in_tiles, sizeof(std::uint32_t) * (*host_in_total + 1))) != hipSuccess) return e;
-  if ((e = hipMalloc(in_stage, sizeof(std::uint32_t) * (*host_in_total + 1))) != hipSuccess) return e;
-  in.list_tiles = *in_tiles;
-  in.list_stage = *in_stage;
-  hipLaunchKernelGGL(fill_pack_kernel, dim3(tile_count), dim3(256), 0, stream, in);
+  if ((e = hipMalloc(t.in_tiles, sizeof(std::uint32_t) * (*t.host_in_total + 1))) != hipSuccess)
+    return e;
+  if ((e = hipMalloc(t.in_stage, sizeof(std::uint32_t) * (*t.host_in_total + 1))) != hipSuccess)
+    return e;
+  in.list_tiles = *t.in_tiles;
+  in.list_stage = *t.in_stage;
+  hipLaunchKernelGGL(fill_pack_kernel, dim3(t.tile_count), dim3(256), 0, stream, in);
   // Out-neighbour lists.
-  hipLaunchKernelGGL(count_kernel, dim3(tile_count), dim3(256), 0, stream, out);
-  hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(1024), 0, stream, counts_scratch,
-                     out_offsets, tile_count);
-  if ((e = hipMemcpyAsync(host_out_total, out_offsets + tile_count, sizeof(std::uint32_t),
+  hipLaunchKernelGGL(count_kernel, dim3(t.tile_count), dim3(256), 0, stream, out);
+  hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(1024), 0, stream, t.counts_scratch,
+                     t.out_offsets, t.tile_count);
+  if ((e = hipMemcpyAsync(t.host_out_total, t.out_offsets + t.tile_count, sizeof(std::uint32_t),
                           hipMemcpyDeviceToHost, stream)) != hipSuccess)
     return e;
   if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
-  if ((e = hipMalloc(out_tiles, sizeof(std::uint32_t) * (*host_out_total + 1))) != hipSuccess) return e;
-  out.list_tiles = *out_tiles;
-  hipLaunchKernelGGL(fill_pack_kernel, dim3(tile_count), dim3(256), 0, stream, out);
+  if ((e = hipMalloc(t.out_tiles, sizeof(std::uint32_t) * (*t.host_out_total + 1))) != hipSuccess)
+    return e;
+  out.list_tiles = *t.out_tiles;
+  hipLaunchKernelGGL(fill_pack_kernel, dim3(t.tile_count), dim3(256), 0, stream, out);
+  // Relaxation streams.
+  const SplitArgs sin{t.tile_vertex_offsets, t.column_offsets, t.csc_pack, t.in_weights,
+                      t.csc_rec, t.csc_perm, t.csc_intra};
+  const SplitArgs sout{t.tile_vertex_offsets, t.row_offsets, t.csr_pack, t.weights,
+                       t.csr_rec, t.csr_perm, t.csr_intra};
+  hipLaunchKernelGGL(split_records_kernel, dim3(t.tile_count), dim3(256), 0, stream, sin);
+  hipLaunchKernelGGL(split_records_kernel, dim3(t.tile_count), dim3(256), 0, stream, sout);
+  return hipGetLastError();
+}
+
+hipError_t refresh_record_weights(const DevGraph& g, const std::uint32_t* csr_perm,
+                                  const std::uint32_t* csc_perm, hipStream_t stream) {
+  const std::uint32_t n = g.edge_count;
+  if (n == 0) return hipSuccess;
+  const dim3 grid((n + 255) / 256);
+  hipLaunchKernelGGL(refresh_weights_kernel, grid, dim3(256), 0, stream,
+                     const_cast<uint2*>(g.csr_rec), csr_perm, g.weights, n);
+  hipLaunchKernelGGL(refresh_weights_kernel, grid, dim3(256), 0, stream,
+                     const_cast<uint2*>(g.csc_rec), csc_perm, g.in_weights, n);
   return hipGetLastError();
 }
 

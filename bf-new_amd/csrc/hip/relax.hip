@@ -1,12 +1,13 @@
 // relax.hip -- the three relaxation engines as tile visits, and their control.
 //
 // Work unit: a slot s = (query q, selected tile T).  One workgroup visits one
-// slot at a time.  Edges of the tile are read once per visit, coalesced, into
-// registers (thread t holds edges t, t+B, t+2B, ... of the tile's contiguous
-// CSC or CSR block); the distances the visit touches are staged in LDS and
-// relaxed there with ds_min_u32 on the order-preserving u32 image of the fp32
-// distance (all distances are +0, finite positive or +inf, so unsigned order
-// equals float order).
+// slot at a time.  The tile's edge records (pack, weight) are read once per
+// visit, coalesced: the internal records (both ends in T) into registers,
+// where they stay for the tile closure (thread t holds records t, t+B, ...),
+// the boundary records streamed in chunks.  The distances the visit touches
+// are staged in LDS and relaxed there with ds_min_u32 on the order-preserving
+// u32 image of the fp32 distance (all distances are +0, finite positive or
+// +inf, so unsigned order equals float order).
 //
 // Semantics (reference engines):
 //   Jacobi pull   batched_jacobi.hip.cpp:466-694 -- two columns, round r reads
@@ -66,29 +67,34 @@ __device__ __forceinline__ void enqueue(const DevBatch& b, std::uint32_t slot,
   }
 }
 
-template <int B, int EPT>
-struct EdgeRegs {
-  std::uint32_t pack[EPT];
-  float w[EPT];
+// Edge records in registers: thread t holds records t, t + B, t + 2B, ... of
+// the range it loaded.
+template <int B, int N>
+struct RecordRegs {
+  std::uint32_t pack[N];
+  float w[N];
 };
 
-// Loads chunk [base, base + B*EPT) of a tile's edge block into registers.
-template <int B, int EPT>
-__device__ __forceinline__ void load_edges(EdgeRegs<B, EPT>& r, const std::uint32_t* pack,
-                                           const float* w, std::uint32_t base,
-                                           std::uint32_t end) {
+// Loads records [base, base + N*B) clipped to `end` (kNone past the end).
+template <int B, int N>
+__device__ __forceinline__ void load_records(RecordRegs<B, N>& r, const uint2* rec,
+                                             std::uint32_t base, std::uint32_t end) {
 #pragma unroll
-  for (int j = 0; j < EPT; ++j) {
+  for (int j = 0; j < N; ++j) {
     const std::uint32_t e = base + j * B + threadIdx.x;
     if (e < end) {
-      r.pack[j] = pack[e];
-      r.w[j] = w[e];
+      const uint2 x = rec[e];
+      r.pack[j] = x.x;
+      r.w[j] = __uint_as_float(x.y);
     } else {
       r.pack[j] = kNone;
       r.w[j] = 0.0f;
     }
   }
 }
+
+// Boundary records per thread per streamed chunk (loads in flight per lane).
+constexpr int kStreamRecords = 8;
 
 __device__ __forceinline__ std::uint32_t pk_k(std::uint32_t p) { return p >> 24; }
 __device__ __forceinline__ std::uint32_t pk_src(std::uint32_t p) { return (p >> 12) & kPackLocalMask; }
@@ -99,7 +105,7 @@ __device__ __forceinline__ std::uint32_t pk_dst(std::uint32_t p) { return p & kP
 // ---------------------------------------------------------------------------
 template <int B, int EPT>
 __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b, std::uint32_t s,
-                           std::uint32_t r, const Lds& l, EdgeRegs<B, EPT>& regs,
+                           std::uint32_t r, const Lds& l, RecordRegs<B, EPT>& internal,
                            unsigned long long& bytes, unsigned long long& edges) {
   const std::uint32_t tid = threadIdx.x;
   const std::uint32_t T = b.tiles[s];
@@ -131,39 +137,47 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
   for (std::uint32_t i = tid; i < npt; i += B) l.own[i] = l.work[i];
   __syncthreads();
 
+  // CSC block of T: internal records [e0, ei), boundary records [ei, e1).
   const std::uint32_t e0 = g.column_offsets[v0];
   const std::uint32_t e1 = g.column_offsets[v0 + npt];
-  const bool single = (e1 - e0) <= static_cast<std::uint32_t>(B * EPT);
+  const std::uint32_t ei = e0 + g.csc_intra[T];
+  const bool resident = ei - e0 <= static_cast<std::uint32_t>(B * EPT);
   if (tid == 0) {
     // edge records (pack + weight) once, the written column once
     bytes += 8ull * (e1 - e0) + 4ull * npt;
     edges += e1 - e0;
   }
   // Pass 1: every admitted in-edge reads the staged read column.
-  for (std::uint32_t base = e0; base < e1; base += B * EPT) {
-    load_edges<B, EPT>(regs, g.csc_pack, g.in_weights, base, e1);
+  auto relax_staged = [&](std::uint32_t p, float w) {
+    const std::uint32_t cb = as_bits(as_float(l.work[l.table[pk_k(p)] + pk_src(p)]) + w);
+    if (cb < l.own[pk_dst(p)]) atomicMin(&l.own[pk_dst(p)], cb);
+  };
+  for (std::uint32_t base = e0; base < ei; base += B * EPT) {
+    load_records<B, EPT>(internal, g.csc_rec, base, ei);
 #pragma unroll
-    for (int j = 0; j < EPT; ++j) {
-      const std::uint32_t p = regs.pack[j];
-      if (p == kNone) continue;
-      const float cand = as_float(l.work[l.table[pk_k(p)] + pk_src(p)]) + regs.w[j];
-      const std::uint32_t cb = as_bits(cand);
-      if (cb < l.own[pk_dst(p)]) atomicMin(&l.own[pk_dst(p)], cb);
-    }
+    for (int j = 0; j < EPT; ++j)
+      if (internal.pack[j] != kNone) relax_staged(internal.pack[j], internal.w[j]);
+  }
+  for (std::uint32_t base = ei; base < e1; base += B * kStreamRecords) {
+    RecordRegs<B, kStreamRecords> chunk;
+    load_records<B, kStreamRecords>(chunk, g.csc_rec, base, e1);
+#pragma unroll
+    for (int j = 0; j < kStreamRecords; ++j)
+      if (chunk.pack[j] != kNone) relax_staged(chunk.pack[j], chunk.w[j]);
   }
   __syncthreads();
   if (b.tile_sweeps == 0) {
-    // Close the tile's internal edges in place (k == 0 sources are in T).
+    // Close the tile's internal edges in place; the internal records stay in
+    // registers when they fit.
     for (;;) {
       bool changed = false;
-      for (std::uint32_t base = e0; base < e1; base += B * EPT) {
-        if (!single) load_edges<B, EPT>(regs, g.csc_pack, g.in_weights, base, e1);
+      for (std::uint32_t base = e0; base < ei; base += B * EPT) {
+        if (!resident) load_records<B, EPT>(internal, g.csc_rec, base, ei);
 #pragma unroll
         for (int j = 0; j < EPT; ++j) {
-          const std::uint32_t p = regs.pack[j];
-          if (p == kNone || pk_k(p) != 0) continue;
-          const float cand = as_float(l.own[pk_src(p)]) + regs.w[j];
-          const std::uint32_t cb = as_bits(cand);
+          const std::uint32_t p = internal.pack[j];
+          if (p == kNone) continue;
+          const std::uint32_t cb = as_bits(as_float(l.own[pk_src(p)]) + internal.w[j]);
           if (cb < l.own[pk_dst(p)]) {
             atomicMin(&l.own[pk_dst(p)], cb);
             changed = true;
@@ -199,7 +213,7 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
 template <int B, int EPT>
 __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b, std::uint32_t s,
                            std::uint32_t r, bool frontier, const Lds& l,
-                           EdgeRegs<B, EPT>& regs, unsigned long long& attempts,
+                           RecordRegs<B, EPT>& internal, unsigned long long& attempts,
                            unsigned long long& successes, unsigned long long& bytes,
                            unsigned long long& edges) {
   const std::uint32_t tid = threadIdx.x;
@@ -226,27 +240,29 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
   }
   __syncthreads();
 
+  // CSR block of T: internal records [e0, ei), boundary records [ei, e1).
   const std::uint32_t e0 = g.row_offsets[v0];
   const std::uint32_t e1 = g.row_offsets[v0 + npt];
-  const bool single = (e1 - e0) <= static_cast<std::uint32_t>(B * EPT);
+  const std::uint32_t ei = e0 + g.csr_intra[T];
+  const bool resident = ei - e0 <= static_cast<std::uint32_t>(B * EPT);
   if (tid == 0) {
     // edge records once, the column (and the snapshot for frontier) once
     bytes += 8ull * (e1 - e0) + 4ull * npt * (frontier ? 2 : 1);
     edges += e1 - e0;
   }
-  if (single) load_edges<B, EPT>(regs, g.csr_pack, g.weights, e0, e1);
+  if (resident) load_records<B, EPT>(internal, g.csr_rec, e0, ei);
   // Internal edges to the tile's local fixed point, from frontier sources.
   for (;;) {
     bool changed = false;
-    for (std::uint32_t base = e0; base < e1; base += B * EPT) {
-      if (!single) load_edges<B, EPT>(regs, g.csr_pack, g.weights, base, e1);
+    for (std::uint32_t base = e0; base < ei; base += B * EPT) {
+      if (!resident) load_records<B, EPT>(internal, g.csr_rec, base, ei);
 #pragma unroll
       for (int j = 0; j < EPT; ++j) {
-        const std::uint32_t p = regs.pack[j];
-        if (p == kNone || pk_k(p) != 0) continue;
+        const std::uint32_t p = internal.pack[j];
+        if (p == kNone) continue;
         const std::uint32_t sv = l.own[pk_src(p)];
         if (!(sv < l.work[pk_src(p)])) continue;
-        const std::uint32_t cb = as_bits(as_float(sv) + regs.w[j]);
+        const std::uint32_t cb = as_bits(as_float(sv) + internal.w[j]);
         if (cb < l.own[pk_dst(p)]) {
           atomicMin(&l.own[pk_dst(p)], cb);
           changed = true;
@@ -255,23 +271,26 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
     }
     if (!__syncthreads_or(changed)) break;
   }
-  // Boundary edges: push into neighbour slots with a global atomic min.
+  // Boundary records, streamed once: push into neighbour slots with a global
+  // atomic min.  The read-first filter is an ordinary (cacheable) load: labels
+  // only fall during a run, so a stale copy is never below the current label
+  // and the filter never drops a decrease.
   unsigned long long att = 0, suc = 0;
-  for (std::uint32_t base = e0; base < e1; base += B * EPT) {
-    if (!single) load_edges<B, EPT>(regs, g.csr_pack, g.weights, base, e1);
+  for (std::uint32_t base = ei; base < e1; base += B * kStreamRecords) {
+    RecordRegs<B, kStreamRecords> chunk;
+    load_records<B, kStreamRecords>(chunk, g.csr_rec, base, e1);
 #pragma unroll
-    for (int j = 0; j < EPT; ++j) {
-      const std::uint32_t p = regs.pack[j];
-      if (p == kNone || pk_k(p) == 0) continue;
+    for (int j = 0; j < kStreamRecords; ++j) {
+      const std::uint32_t p = chunk.pack[j];
+      if (p == kNone) continue;
       const std::uint32_t sv = l.own[pk_src(p)];
       if (!(sv < l.work[pk_src(p)])) continue;
       const std::uint32_t ns = l.table[pk_k(p)];
       if (ns == kNone) continue;
-      const std::uint32_t cb = as_bits(as_float(sv) + regs.w[j]);
+      const std::uint32_t cb = as_bits(as_float(sv) + chunk.w[j]);
       std::uint32_t* target = b.dist + b.slot_dist[ns] + pk_dst(p);
       ++att;
-      // Read first: most candidates lose once labels settle.
-      if (cb < load_agent(target)) {
+      if (cb < *target) {
         const std::uint32_t old = atomic_min_agent(target, cb);
         if (cb < old) {
           ++suc;
@@ -314,7 +333,7 @@ __device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch&
                               std::uint32_t n, const Lds& l) {
   const std::uint32_t li = r % 3;
   const std::uint32_t chunk = b.items_per_claim ? b.items_per_claim : 1u;
-  EdgeRegs<B, EPT> regs;
+  RecordRegs<B, EPT> regs;
   unsigned long long attempts = 0, successes = 0, items = 0, bytes = 0, edges = 0;
   for (;;) {
     if (threadIdx.x == 0) l.flags[0] = atomic_add_agent(&b.control->claim[li], chunk);
@@ -553,8 +572,9 @@ hipError_t occupancy_relax(bool persistent, std::size_t lds_bytes, int* blocks_p
 }
 
 // Block sizes the engine is instantiated for, each with the registers per
-// thread that hold a tile's edge block (B * EPT edges per register load).
-#define BFN_FOR_EACH_SHAPE(X) X(64, 16) X(128, 16) X(256, 24) X(512, 12) X(1024, 8)
+// thread that hold a tile's internal records (B * EPT = 2048: 512-vertex
+// tiles with four internal out-edges per vertex stay register-resident).
+#define BFN_FOR_EACH_SHAPE(X) X(64, 32) X(128, 16) X(256, 8) X(512, 4) X(1024, 2)
 
 hipError_t relax_launch(std::uint32_t block, bool persistent, bool cooperative,
                         std::uint32_t grid, std::size_t lds_bytes, std::uint32_t max_v,
