@@ -54,6 +54,25 @@ def load_tuning(workload):
     return None
 
 
+def profiled_traffic(workload, tune):
+    """HBM bytes per step of the relaxation kernels, measured with rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes over this workload and configuration and
+    recorded in profiles/relax_traffic.json (None when the recorded
+    configuration is not the one being run)."""
+    path = os.path.join(ROOT, "profiles", "relax_traffic.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        entry = json.load(f).get(workload)
+    if not entry:
+        return None, None
+    cfg = entry["config"]
+    keys = ("engine", "block", "sweeps", "boundary_mode", "use_graph")
+    if any(int(cfg.get(k, 0)) != int(tune.get(k, 0)) for k in keys):
+        return None, None
+    return entry["hbm_bytes_per_step"], entry["source"]
+
+
 def query_class(q):
     if q.unbounded or q.box_side > 32:
         return "large"
@@ -69,7 +88,7 @@ def make_batch(hg, cfg):
 
 def apply_tuning(batch, qs, tune_entry):
     t = dict(engine=bfn.JACOBI_PULL, block=256, sweeps=0, control=bfn.PERSISTENT,
-             classes=None, items_per_claim=1, use_graph=0, graph_rounds=0)
+             classes=None, items_per_claim=1, use_graph=0, graph_rounds=0, boundary_mode=0)
     if tune_entry:
         t.update(tune_entry)
     if t.get("classes"):
@@ -169,6 +188,108 @@ def plain_vs_batched(dev, steps):
                 batched_over_best_plain=batched / best_plain)
 
 
+SERVING = dict(name="c4_serving_64x64x256", tiles=(64, 64, 256), seed=2, query_seed=44,
+               rates=(2000, 10000, 50000), max_requests=4000)
+
+
+def percentile(values, p):
+    """Nearest-rank percentile (the reference's engine_shootout.cpp:1538-1567 rule)."""
+    v = sorted(values)
+    if not v:
+        return None
+    k = max(1, int(np.ceil(p / 100.0 * len(v))))
+    return v[k - 1]
+
+
+def serving(dev):
+    """Poisson arrivals into the continuous-batching server on the 64x64x256
+    graph, box mix 70/25/5; latency = response time minus scheduled arrival
+    (host monotonic clock), nearest-rank p50/p99."""
+    hg = bfn.HostGraph.synthetic(*SERVING["tiles"], SERVING["seed"], host_graph=False)
+    dg = bfn.DeviceGraph(dev, hg.layout())
+    out = {"workload": SERVING["name"], "mix": "70% side 4-8, 25% 16-32, 5% 64 or unbounded"}
+    rng = np.random.default_rng(7)
+    for rate in SERVING["rates"]:
+        n = min(SERVING["max_requests"], rate)
+        qs = hg.synthetic_queries(n, seed=SERVING["query_seed"] + rate, mix=1)
+        tiles = [hg.query_tiles(q) for q in qs]
+        gaps = rng.exponential(1.0 / rate, n)
+        offsets_ns = np.cumsum(gaps) * 1e9
+        server = bfn.Server(dev, dg, budget_bytes=8 << 30, max_active_queries=1024,
+                            rounds_per_chunk=2, engine=bfn.FRONTIER_PUSH, tile_sweeps=0)
+        got = {}
+        try:
+            t0 = time.monotonic_ns()
+            i = 0
+            while i < n:
+                now = time.monotonic_ns() - t0
+                while i < n and offsets_ns[i] <= now:
+                    server.submit(i, qs[i].source, qs[i].target, tiles[i])
+                    i += 1
+                for r in server.poll(capacity=1024, timeout_ms=0):
+                    got[r["request_id"]] = r
+                if i < n:  # sleep towards the next arrival, at most 0.2 ms
+                    wait_ns = offsets_ns[i] - (time.monotonic_ns() - t0)
+                    if wait_ns > 0:
+                        time.sleep(min(wait_ns, 2e5) / 1e9)
+            deadline = time.time() + 120
+            while len(got) < n and time.time() < deadline:
+                for r in server.poll(capacity=1024, timeout_ms=50):
+                    got[r["request_id"]] = r
+            t_end = time.monotonic_ns()
+        finally:
+            server.close()
+        lat = [(got[k]["complete_ns"] - (t0 + offsets_ns[k])) / 1e6 for k in got]
+        out[str(rate)] = {"requests": n, "completed": len(got),
+                          "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
+                          "throughput_qps": len(got) / ((t_end - t0) / 1e9)}
+        log(f"serving {rate}/s:", json.dumps(out[str(rate)]))
+    dg.close()
+    return out
+
+
+C5 = dict(name="c5_negotiated_256x256x512_q1024", tiles=(256, 256, 512), seed=5, queries=1024,
+          query_seed=55, side=(4, 64), unbounded=0.0, rounds=4)
+
+
+def negotiated(dev, tune):
+    """Negotiated-cost rounds on the 256x256x512 graph: 1024 queries routed
+    four times; after each round every edge u->v is re-costed on device to
+    base[v] * (1 + 0.5 * occupancy[v]) from the round's compact paths."""
+    t = time.perf_counter()
+    hg = bfn.HostGraph.synthetic(*C5["tiles"], C5["seed"], host_graph=False)
+    log(f"negotiated: graph generated in {time.perf_counter() - t:.1f}s")
+    dg = bfn.DeviceGraph(dev, hg.layout())
+    base = hg.view()["node_costs"]
+    dg.update_node_costs(base)
+    qs, batch = make_batch(hg, C5)
+    ws = bfn.Workspace(dev)
+    opts = bfn.run_options(engine=tune["engine"], control_mode=tune["control"],
+                           block_size=tune["block"])
+    rounds = []
+    for r in range(C5["rounds"]):
+        ws.submit(dg, batch, opts, bfn.tuning(tile_sweeps=tune["sweeps"], count_relaxations=1,
+                                              items_per_claim=tune["items_per_claim"],
+                                              boundary_mode=tune["boundary_mode"]))
+        st, per, _ = ws.status()
+        relax_ms = ws.timing()[0]
+        relaxations = sum(p.relaxations for p in per)
+        t = time.perf_counter()
+        paths = ws.paths()
+        paths_ms = (time.perf_counter() - t) * 1e3
+        t = time.perf_counter()
+        dg.update_node_costs_from_workspace(ws)
+        update_ms = (time.perf_counter() - t) * 1e3
+        rounds.append({"converged": int(st.converged), "relaxations": relaxations,
+                       "relax_ms": relax_ms, "relax_per_s": relaxations / (relax_ms * 1e-3),
+                       "paths_ms": paths_ms, "cost_update_ms": update_ms,
+                       "complete_paths": sum(1 for p in paths if p["reconstruction"] == 0)})
+        log(f"negotiated round {r}:", json.dumps(rounds[-1]))
+    ws.close()
+    dg.close()
+    return {"workload": C5["name"], "engine": bfn.ENGINE_NAMES[tune["engine"]], "rounds": rounds}
+
+
 def copy_calibration():
     """Three copies of 1 GiB between two device buffers: a kernel of known
     traffic (1 GiB read, 1 GiB written per dispatch) against which the HBM
@@ -190,6 +311,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-budget-s", type=float, default=10.0)
+    ap.add_argument("--negotiated", action="store_true",
+                    help="also run the negotiated-cost rounds on the 256x256x512 graph")
+    ap.add_argument("--serving", action="store_true",
+                    help="also run the Poisson-arrival serving measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--engine", type=int, default=None, help="override the tuned engine")
@@ -225,7 +350,8 @@ def main():
                                          block_size=tune["block"], instrumentation=1),
               bfn.tuning(tile_sweeps=tune["sweeps"], count_relaxations=1,
                          items_per_claim=tune["items_per_claim"], use_graph=tune["use_graph"],
-                         graph_rounds=tune["graph_rounds"]))
+                         graph_rounds=tune["graph_rounds"],
+                         boundary_mode=tune["boundary_mode"]))
     st, per, _ = ws.status()
     if not st.converged:
         raise SystemExit("batch did not converge")
@@ -235,7 +361,8 @@ def main():
     ws.submit(dg, batch, opts, bfn.tuning(tile_sweeps=tune["sweeps"],
                                           items_per_claim=tune["items_per_claim"],
                                           use_graph=tune["use_graph"],
-                                          graph_rounds=tune["graph_rounds"]))
+                                          graph_rounds=tune["graph_rounds"],
+                                          boundary_mode=tune["boundary_mode"]))
     elapsed, relax_ms = timed_steps(ws, args.steps, args.warmup)
     value = relax_per_step * args.steps / elapsed
     ms_per_step = elapsed * 1e3 / args.steps
@@ -243,6 +370,17 @@ def main():
     achieved = counters.algorithmic_bytes / (kernel_ms * 1e-3) / 1e9
     log(f"{value / 1e9:.2f} G relax/s, {ms_per_step:.2f} ms/step, relax kernel {kernel_ms:.3f} ms, "
         f"{counters.tile_visits} tile visits, {counters.rounds_executed} rounds")
+    traffic, traffic_source = profiled_traffic(C3["name"], tune)
+    cycles = [int(c) for c in counters.phase_cycles]
+    visit_profile = {
+        "tile_visits": counters.tile_visits,
+        "closure_sweeps_per_visit": counters.closure_sweeps / max(1, counters.tile_visits),
+        "phase_share": {n: c / max(1, sum(cycles)) for n, c in zip(bfn.PHASE_NAMES, cycles)},
+        "cycles_per_visit": sum(cycles) / max(1, counters.tile_visits),
+        "atomic_attempts": counters.atomic_attempts,
+        "atomic_successes": counters.atomic_successes,
+    }
+    log("visit profile (instrumented run):", json.dumps(visit_profile))
 
     result = {
         "metric": "batched SSSP edge relaxations/s + p99 query latency, 1 MI355X, vs plain mode",
@@ -261,20 +399,26 @@ def main():
                    "queries": C3["queries"], "box_side": list(C3["side"]),
                    "unbounded_fraction": C3["unbounded"], "engine": bfn.ENGINE_NAMES[tune["engine"]],
                    "per_class_engines": tune.get("classes"), "block": tune["block"],
-                   "tile_sweeps": tune["sweeps"],
+                   "tile_sweeps": tune["sweeps"], "boundary_mode": tune["boundary_mode"],
                    "control": bfn.CONTROL_NAMES[tune["control"]] +
                    (f"+hipgraph({tune['graph_rounds']} rounds)" if tune["use_graph"] else ""),
                    "relaxations_per_step": relax_per_step},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_source,
                      "kernel": "relax (tile visits)", "kernel_ms": kernel_ms,
                      "algorithmic_bytes_per_launch": counters.algorithmic_bytes,
                      "edges_relaxed_per_launch": counters.edges_relaxed},
+        "visit_profile": visit_profile,
     }
     if not args.no_plain:
         pm = plain_vs_batched(dev, max(3, args.steps // 2))
         result["plain_mode"] = pm
         log("plain vs batched (C2):", json.dumps(pm))
+    if args.serving:
+        result["serving"] = serving(dev)
+    if args.negotiated:
+        result["negotiated"] = negotiated(dev, dict(tune, control=bfn.PERSISTENT))
     if not args.no_cpu_baseline:
         v = hg.layout_arrays()
         view = dict(vertex_count=v["vertex_count"], tile_count=v["tile_count"],

@@ -78,7 +78,8 @@ class QueryStatus(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("tile_sweeps", C.c_uint32), ("items_per_claim", C.c_uint32),
                 ("graph_rounds", C.c_uint32), ("use_graph", C.c_uint32),
-                ("count_relaxations", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("count_relaxations", C.c_uint32), ("boundary_mode", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
 
 
 _P32 = C.POINTER(C.c_uint32)
@@ -154,10 +155,14 @@ class Response(C.Structure):
                 ("submit_ns", C.c_double), ("complete_ns", C.c_double)]
 
 
+PHASE_NAMES = ("setup", "internal", "closure", "boundary", "writeback")
+
+
 class EngineCounters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("tile_visits", "edges_relaxed", "algorithmic_bytes",
                                           "atomic_attempts", "atomic_successes", "maximum_list",
-                                          "rounds_executed", "reserved")]
+                                          "rounds_executed", "closure_sweeps")] + [
+        ("phase_cycles", C.c_uint64 * len(PHASE_NAMES))]
 
 
 VERTEX_DTYPE = np.dtype([("x", "<i4"), ("y", "<i4"), ("has_location", "<u4"),
@@ -289,11 +294,13 @@ def run_options(engine=JACOBI_PULL, control_mode=PERSISTENT, rounds_per_chunk=8,
     return o
 
 
-def tuning(tile_sweeps=1, items_per_claim=1, graph_rounds=0, use_graph=0, count_relaxations=0):
+def tuning(tile_sweeps=1, items_per_claim=1, graph_rounds=0, use_graph=0, count_relaxations=0,
+           boundary_mode=0):
     t = Tuning()
     hip_lib().bfn_tuning_default(C.byref(t))
     t.tile_sweeps, t.items_per_claim = tile_sweeps, items_per_claim
     t.graph_rounds, t.use_graph, t.count_relaxations = graph_rounds, use_graph, count_relaxations
+    t.boundary_mode = boundary_mode
     return t
 
 
@@ -623,6 +630,55 @@ class Workspace:
         if self.h:
             hip_lib().bfn_workspace_free(self.h)
             self.h = None
+
+
+class Server:
+    """Continuous-batching route server over a resident graph (bfn_server_*).
+    Requests join the running batch between chunks of rounds and come back
+    as soon as their own work has drained."""
+
+    def __init__(self, device: Device, graph: DeviceGraph, budget_bytes=1 << 30,
+                 max_active_queries=256, rounds_per_chunk=4, engine=FRONTIER_PUSH, tile_sweeps=0,
+                 block_size=256, want_paths=1):
+        o = ServerOptions()
+        o.budget_bytes, o.max_active_queries = budget_bytes, max_active_queries
+        o.rounds_per_chunk, o.engine, o.tile_sweeps = rounds_per_chunk, engine, tile_sweeps
+        o.block_size, o.want_paths = block_size, want_paths
+        self.device, self.graph = device, graph
+        self.h = C.c_void_p()
+        _check(hip_lib().bfn_server_create(device.h, graph.h, C.byref(o), C.byref(self.h)))
+
+    def submit(self, request_id, source, target, tiles):
+        t = np.ascontiguousarray(tiles, np.uint32)
+        r = Request()
+        r.request_id, r.source, r.target, r.tile_count = request_id, source, target, len(t)
+        r.selected_tiles = _p32(t)
+        _check(hip_lib().bfn_server_submit(self.h, C.byref(r)))
+
+    def poll(self, capacity=1024, timeout_ms=100):
+        buf = (Response * capacity)()
+        n = C.c_uint32()
+        _check(hip_lib().bfn_server_poll(self.h, buf, capacity, C.byref(n), timeout_ms))
+        return [dict(request_id=buf[i].request_id, distance=buf[i].distance,
+                     path_length=buf[i].path_length, reached=buf[i].reached,
+                     submit_ns=buf[i].submit_ns, complete_ns=buf[i].complete_ns)
+                for i in range(n.value)]
+
+    def stats(self):
+        a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(hip_lib().bfn_server_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return dict(launches=a.value, admitted=b.value, completed=c.value)
+
+    def close(self):
+        if self.h:
+            hip_lib().bfn_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def gpu_available() -> bool:

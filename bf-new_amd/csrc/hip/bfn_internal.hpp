@@ -11,6 +11,8 @@
 //                       each tile's incoming edges, the tile itself first)
 //                       out_nbr_offsets[T+1] / out_nbr_tiles[] (same, outgoing)
 //                       in_nbr_stage[]  LDS float offset of each in-neighbour slab
+//                       out_nbr_stage[] LDS float offset of each out-neighbour
+//                       proposal slab (k >= 1; the tile itself has none)
 //     packed edges    : csc_pack[E] = k << 24 | src_local << 12 | dst_local
 //                       (k indexes the destination tile's in-neighbour list)
 //                       csr_pack[E] = k << 24 | src_local << 12 | dst_local
@@ -44,6 +46,11 @@ constexpr std::uint32_t kInfBits = 0x7F800000u;  // +inf; non-negative floats or
 constexpr std::uint32_t kMaxTileVertices = 4096;  // 12-bit local ids
 constexpr std::uint32_t kMaxNeighborTiles = 255;  // 8-bit neighbour index
 constexpr std::uint32_t kPackLocalMask = 0xFFFu;
+constexpr int kPhaseCount = 5;  // visit phases timed by instrumented runs
+// LDS table of a visit: neighbour slots [0, 256), pushed-into marks
+// [256, 512), proposal slab offsets [512, 768); then the flag words.
+constexpr std::uint32_t kLdsTableWords = 1024;
+constexpr std::uint32_t kLdsFlagWords = 8;
 
 enum Engine : std::uint32_t { kJacobiPull = 0, kDensePush = 1, kFrontierPush = 2 };
 
@@ -64,7 +71,7 @@ struct DevGraph {
   std::uint32_t max_stage_floats;
   std::uint32_t max_in_edges;
   std::uint32_t max_out_edges;
-  std::uint32_t pad;
+  std::uint32_t max_out_stage_floats;  // largest out-neighbour proposal area (k >= 1)
   const std::uint32_t* owner_tiles;
   const std::uint32_t* tile_vertex_offsets;
   const std::uint32_t* row_offsets;
@@ -79,6 +86,7 @@ struct DevGraph {
   const std::uint32_t* in_nbr_stage;
   const std::uint32_t* out_nbr_offsets;
   const std::uint32_t* out_nbr_tiles;
+  const std::uint32_t* out_nbr_stage;   // LDS offset of each out-neighbour proposal slab
   const std::uint32_t* csc_pack;
   const std::uint32_t* csr_pack;
   // Relaxation streams: each tile's block as [internal | boundary] records
@@ -108,6 +116,8 @@ struct DevControl {
   unsigned long long max_list;
   unsigned long long edges_relaxed;      // edge records evaluated by visits
   unsigned long long algorithmic_bytes;  // modelled HBM bytes of the visits
+  unsigned long long closure_sweeps;     // tile-closure sweeps (one per block pass)
+  unsigned long long phase_cycles[kPhaseCount];  // summed over workgroups
 };
 
 struct DevBatch {
@@ -117,6 +127,8 @@ struct DevBatch {
   std::uint32_t tile_sweeps;      // Jacobi: 1 point, 0 closure
   std::uint32_t items_per_claim;
   std::uint32_t instrumentation;
+  std::uint32_t boundary_mode;    // push: 0 LDS proposals when they fit, 1 direct atomics
+  std::uint32_t pad;
   const std::uint32_t* slot_begin;      // [Q]   first slot of q
   const std::uint32_t* slot_end;        // [Q]   one past q's last slot
   const std::uint32_t* tiles;           // [S]   tile of slot

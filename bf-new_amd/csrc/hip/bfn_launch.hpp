@@ -23,8 +23,8 @@ hipError_t relax_launch(std::uint32_t block, bool persistent, bool cooperative,
                         std::uint32_t grid, std::size_t lds_bytes, std::uint32_t max_v,
                         std::uint32_t work_words, const DevGraph& g, const DevBatch& b,
                         hipStream_t stream);
-hipError_t relax_occupancy(std::uint32_t block, bool persistent, std::size_t lds_bytes,
-                           int* blocks_per_cu);
+hipError_t relax_occupancy(std::uint32_t block, bool persistent, bool instrumented,
+                           std::size_t lds_bytes, int* blocks_per_cu);
 hipError_t prep_launch(const DevGraph& g, const DevBatch& b, std::uint64_t pool_words,
                        hipStream_t stream);
 // Inputs and outputs of the tile-image derivation (graph.hip).
@@ -48,6 +48,7 @@ struct TileImageArgs {
   std::uint32_t** in_tiles;   // allocated here
   std::uint32_t** in_stage;   // allocated here
   std::uint32_t** out_tiles;  // allocated here
+  std::uint32_t** out_stage;  // allocated here
   std::uint32_t* csc_pack;
   std::uint32_t* csr_pack;
   uint2* csc_rec;

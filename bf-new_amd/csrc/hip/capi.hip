@@ -99,6 +99,12 @@ struct PinnedBuffer {
 
 constexpr std::size_t align_up(std::size_t x, std::size_t a) { return (x + a - 1) / a * a; }
 
+// LDS words of one relaxation workgroup: own + orig labels, the work area
+// (staged slabs, or snapshot + proposals), the table and the flags.
+constexpr std::size_t lds_words(std::uint32_t max_v, std::uint32_t work_words) {
+  return 2ull * max_v + work_words + kLdsTableWords + kLdsFlagWords;
+}
+
 bool valid_block(std::uint32_t b) {
   return b == 64 || b == 128 || b == 256 || b == 512 || b == 1024;
 }
@@ -121,12 +127,14 @@ struct bfn_graph_s {
   std::uint32_t* in_tiles = nullptr;
   std::uint32_t* in_stage = nullptr;
   std::uint32_t* out_tiles = nullptr;
+  std::uint32_t* out_stage = nullptr;
   std::vector<std::uint32_t> h_tile_off, h_in_off, h_out_off;
   bfn_graph_info info{};
   ~bfn_graph_s() {
     if (in_tiles) (void)hipFree(in_tiles);
     if (in_stage) (void)hipFree(in_stage);
     if (out_tiles) (void)hipFree(out_tiles);
+    if (out_stage) (void)hipFree(out_stage);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -350,6 +358,7 @@ int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_gra
   ta.in_tiles = &g->in_tiles;
   ta.in_stage = &g->in_stage;
   ta.out_tiles = &g->out_tiles;
+  ta.out_stage = &g->out_stage;
   ta.csc_pack = g->csc_pack.as<std::uint32_t>();
   ta.csr_pack = g->csr_pack.as<std::uint32_t>();
   ta.csc_rec = g->csc_rec.as<uint2>();
@@ -386,6 +395,7 @@ int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_gra
   v.max_stage_floats = std::max(stats[1], 1u);
   v.max_in_edges = stats[2];
   v.max_out_edges = stats[5];
+  v.max_out_stage_floats = stats[6];
   v.owner_tiles = g->owner.as<std::uint32_t>();
   v.tile_vertex_offsets = g->tile_off.as<std::uint32_t>();
   v.row_offsets = g->row_off.as<std::uint32_t>();
@@ -400,6 +410,7 @@ int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_gra
   v.in_nbr_stage = g->in_stage;
   v.out_nbr_offsets = g->out_off.as<std::uint32_t>();
   v.out_nbr_tiles = g->out_tiles;
+  v.out_nbr_stage = g->out_stage;
   v.csc_pack = g->csc_pack.as<std::uint32_t>();
   v.csr_pack = g->csr_pack.as<std::uint32_t>();
   v.csc_rec = g->csc_rec.as<uint2>();
@@ -422,7 +433,7 @@ int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_gra
   // reference image + derived tile lists + packs (4 B) + records (8 B) +
   // record permutations (4 B) per edge and orientation
   info.device_bytes = 4ull * (2ull * V + 2 + T + 1 + 6ull * E + 2ull * (g->Rcsr + g->Rcsc + 2) +
-                              4ull * (T + 1) + 2ull * in_total + out_total) +
+                              4ull * (T + 1) + 2ull * in_total + 2ull * out_total) +
                       2ull * E * (4 + 8 + 4);
   *out = g;
   return BFN_OK;
@@ -574,6 +585,7 @@ struct BatchPlanHost {
   std::vector<std::uint32_t> slot_query, slot_in_base, slot_out_base, engine;
   std::vector<unsigned long long> slot_dist, frame_base;
   bool any_jacobi = false;
+  bool any_push = false;
 };
 
 int32_t plan_batch(bfn_graph g, const bfn_batch_desc* d, const bfn_run_options* o,
@@ -631,6 +643,7 @@ int32_t plan_batch(bfn_graph g, const bfn_batch_desc* d, const bfn_run_options* 
   for (std::uint32_t q = 0; q < Q; ++q) {
     p.engine[q] = d->engine_by_query ? d->engine_by_query[q] : o->engine;
     p.any_jacobi |= p.engine[q] == kJacobiPull;
+    p.any_push |= p.engine[q] != kJacobiPull;
     p.frame_base[q] = frames;
     for (std::uint32_t s = d->tile_offsets[q]; s < d->tile_offsets[q + 1]; ++s) {
       const std::uint32_t T = d->selected_tiles[s];
@@ -670,6 +683,7 @@ void bind_view(bfn_workspace ws, const BatchPlanHost& p, const std::size_t* seg,
   b.tile_sweeps = t->tile_sweeps;
   b.items_per_claim = std::max(1u, t->items_per_claim);
   b.instrumentation = o->instrumentation;
+  b.boundary_mode = t->boundary_mode;
   b.slot_begin = reinterpret_cast<const std::uint32_t*>(m + seg[0]);
   b.slot_end = b.slot_begin + 1;
   b.tiles = reinterpret_cast<const std::uint32_t*>(m + seg[1]);
@@ -829,6 +843,8 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
   if (int32_t rc = validate_options(o)) return rc;
   bfn_tuning t;
   if (tuning) t = *tuning; else bfn_tuning_default(&t);
+  if (t.tile_sweeps > 1 || t.boundary_mode > 1 || t.use_graph > 1)
+    return fail(BFN_E_INVALID_ARGUMENT, "invalid tuning (tile_sweeps, boundary_mode, use_graph)");
   BatchPlanHost p;
   if (int32_t rc = plan_batch(graph, d, o, p)) return rc;
   BFN_HIP(hipSetDevice(ws->device->ordinal));
@@ -900,15 +916,23 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
   const DevGraph& gv = graph->view;
   ws->block = o->block_size;
   ws->max_v = gv.max_tile_vertices;
-  ws->work_words = std::max(p.any_jacobi ? gv.max_stage_floats : 0u, gv.max_tile_vertices);
-  ws->lds_bytes = 4ull * (2ull * ws->max_v + ws->work_words + 512 + 8);
   const std::size_t lds_limit = ws->device->props.sharedMemPerBlock;
+  // Push engines stage their boundary proposals after the snapshot when the
+  // LDS holds them; otherwise they push with direct global atomics.
+  const bool proposals = p.any_push && t.boundary_mode == 0 &&
+                         lds_words(ws->max_v, gv.max_tile_vertices + gv.max_out_stage_floats) * 4ull <=
+                             lds_limit;
+  if (!proposals) ws->view.boundary_mode = 1;
+  ws->work_words = std::max({p.any_jacobi ? gv.max_stage_floats : 0u, gv.max_tile_vertices,
+                             proposals ? gv.max_tile_vertices + gv.max_out_stage_floats : 0u});
+  ws->lds_bytes = 4ull * lds_words(ws->max_v, ws->work_words);
   if (ws->lds_bytes > lds_limit)
     return fail(BFN_E_NOT_REPRESENTABLE, "tile neighbourhood exceeds the LDS budget (" +
                                              std::to_string(ws->lds_bytes) + " > " +
                                              std::to_string(lds_limit) + " bytes)");
   int per_cu = 0;
-  BFN_HIP(relax_occupancy(ws->block, o->control_mode == 0, ws->lds_bytes, &per_cu));
+  BFN_HIP(relax_occupancy(ws->block, o->control_mode == 0, o->instrumentation != 0, ws->lds_bytes,
+                          &per_cu));
   if (per_cu <= 0) return fail(BFN_E_NOT_REPRESENTABLE, "relaxation kernel cannot be resident");
   const std::uint32_t cus = static_cast<std::uint32_t>(ws->device->props.multiProcessorCount);
   std::uint32_t blocks = static_cast<std::uint32_t>(per_cu) * cus;
@@ -1128,6 +1152,8 @@ int32_t bfn_batch_counters(bfn_workspace ws, bfn_engine_counters* out) {
   out->atomic_successes = c->atomic_successes;
   out->maximum_list = c->max_list;
   out->rounds_executed = c->rounds_executed;
+  out->closure_sweeps = c->closure_sweeps;
+  for (int p = 0; p < kPhaseCount; ++p) out->phase_cycles[p] = c->phase_cycles[p];
   return BFN_OK;
 }
 

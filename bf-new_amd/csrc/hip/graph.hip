@@ -38,11 +38,12 @@ struct DeriveArgs {
   std::uint32_t* counts;               // [T] distinct neighbour tiles (incl. T)
   const std::uint32_t* list_offsets;   // [T+1] (fill/pack passes)
   std::uint32_t* list_tiles;
-  std::uint32_t* list_stage;           // nullable (CSR has no staging)
+  std::uint32_t* list_stage;           // LDS slab offsets in list order
   std::uint32_t* pack;                 // [E]
   std::uint32_t* stats;                // [0] error, [1] max stage floats, [2] max CSC edges
                                        // per tile, [3] max neighbours, [4] zero weight,
-                                       // [5] max CSR edges per tile
+                                       // [5] max CSR edges per tile, [6] max proposal
+                                       // floats (CSR stage)
   const float* weights;
   bool csc;
 };
@@ -125,12 +126,14 @@ __global__ void fill_pack_kernel(DeriveArgs a) {
   if (n > kMaxNeighborTiles || a.list_offsets[T + 1] - base != n) return;
   for (std::uint32_t i = tid; i < n; i += blockDim.x) a.list_tiles[base + i] = list[i];
   if (a.list_stage && tid == 0) {
+    // CSC: staged read slabs of every in-neighbour, T's own first (offset 0).
+    // CSR: proposal slabs of the other out-neighbours only (k >= 1).
     std::uint32_t off = 0;
     for (std::uint32_t i = 0; i < n; ++i) {
       a.list_stage[base + i] = off;
-      off += a.tile_vertex_offsets[list[i] + 1] - a.tile_vertex_offsets[list[i]];
+      if (a.csc || i > 0) off += a.tile_vertex_offsets[list[i] + 1] - a.tile_vertex_offsets[list[i]];
     }
-    atomicMax(&a.stats[1], off);
+    atomicMax(&a.stats[a.csc ? 1 : 6], off);
   }
   const std::uint32_t v0 = a.tile_vertex_offsets[T], v1 = a.tile_vertex_offsets[T + 1];
   for (std::uint32_t v = v0 + tid; v < v1; v += blockDim.x) {
@@ -280,7 +283,10 @@ hipError_t derive_tile_image(const TileImageArgs& t, hipStream_t stream) {
   if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
   if ((e = hipMalloc(t.out_tiles, sizeof(std::uint32_t) * (*t.host_out_total + 1))) != hipSuccess)
     return e;
+  if ((e = hipMalloc(t.out_stage, sizeof(std::uint32_t) * (*t.host_out_total + 1))) != hipSuccess)
+    return e;
   out.list_tiles = *t.out_tiles;
+  out.list_stage = *t.out_stage;
   hipLaunchKernelGGL(fill_pack_kernel, dim3(t.tile_count), dim3(256), 0, stream, out);
   // Relaxation streams.
   const SplitArgs sin{t.tile_vertex_offsets, t.column_offsets, t.csc_pack, t.in_weights,

@@ -32,14 +32,16 @@
 
 namespace bfn {
 
-// LDS carve-up (u32 words): [own | orig | work | table(512) | flags(8)], with
-// `work` = staged neighbour slabs (pull) or the pushed snapshot (push), and
-// `table` = per-neighbour LDS offsets (pull) or neighbour slots followed by
-// per-neighbour "pushed into" marks (push).
+// LDS carve-up (u32 words): [own | orig | work | table | flags], with
+// `work` = staged neighbour slabs (pull), or the pushed snapshot followed by
+// the boundary proposal slabs (push), and `table` = per-neighbour LDS offsets
+// (pull) or neighbour slots, "pushed into" marks and proposal slab offsets
+// (push).
 struct Lds {
   std::uint32_t* own;
   std::uint32_t* orig;
   std::uint32_t* work;
+  std::uint32_t* prop;
   std::uint32_t* table;
   std::uint32_t* flags;
 };
@@ -50,8 +52,9 @@ __device__ __forceinline__ Lds carve(std::uint32_t* base, std::uint32_t max_v,
   l.own = base;
   l.orig = base + max_v;
   l.work = base + 2 * max_v;
+  l.prop = l.work + max_v;
   l.table = l.work + work_words;
-  l.flags = l.table + 512;
+  l.flags = l.table + kLdsTableWords;
   return l;
 }
 
@@ -93,8 +96,35 @@ __device__ __forceinline__ void load_records(RecordRegs<B, N>& r, const uint2* r
   }
 }
 
-// Boundary records per thread per streamed chunk (loads in flight per lane).
-constexpr int kStreamRecords = 8;
+// Boundary records per thread per streamed chunk (loads in flight per lane):
+// more for the large blocks, fewer where registers bound the occupancy.
+constexpr int stream_records(int block) { return block >= 512 ? 8 : 4; }
+
+// Per-workgroup counters of the visits, compiled in for instrumented runs
+// only (I = true; otherwise every counter is dead code).  Phase clocks are
+// shader-clock cycles read by thread 0 at phase boundaries: 0 setup (tables,
+// label staging), 1 internal records (pull: first pass), 2 tile closure,
+// 3 boundary records, 4 write-back and enqueue.
+template <bool I>
+struct VisitStats {
+  unsigned long long attempts = 0, successes = 0, bytes = 0, edges = 0, items = 0, sweeps = 0;
+  unsigned long long phase[kPhaseCount] = {};
+  long long t = 0;
+  __device__ __forceinline__ void start() {
+    if constexpr (I) {
+      if (threadIdx.x == 0) t = clock64();
+    }
+  }
+  __device__ __forceinline__ void mark(int p) {
+    if constexpr (I) {
+      if (threadIdx.x == 0) {
+        const long long now = clock64();
+        phase[p] += static_cast<unsigned long long>(now - t);
+        t = now;
+      }
+    }
+  }
+};
 
 __device__ __forceinline__ std::uint32_t pk_k(std::uint32_t p) { return p >> 24; }
 __device__ __forceinline__ std::uint32_t pk_src(std::uint32_t p) { return (p >> 12) & kPackLocalMask; }
@@ -103,11 +133,12 @@ __device__ __forceinline__ std::uint32_t pk_dst(std::uint32_t p) { return p & kP
 // ---------------------------------------------------------------------------
 // Jacobi pull visit.
 // ---------------------------------------------------------------------------
-template <int B, int EPT>
+template <int B, int EPT, bool I>
 __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b, std::uint32_t s,
                            std::uint32_t r, const Lds& l, RecordRegs<B, EPT>& internal,
-                           unsigned long long& bytes, unsigned long long& edges) {
+                           VisitStats<I>& st) {
   const std::uint32_t tid = threadIdx.x;
+  st.start();
   const std::uint32_t T = b.tiles[s];
   const std::uint32_t q = b.slot_query[s];
   const std::uint32_t v0 = g.tile_vertex_offsets[T];
@@ -128,7 +159,7 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
     if (ns == kNone) {
       for (std::uint32_t i = tid; i < n; i += B) dst[i] = kInfBits;
     } else {
-      if (tid == 0) bytes += 4ull * n;
+      if (tid == 0) st.bytes += 4ull * n;
       const std::uint32_t* src = b.dist + b.slot_dist[ns] + static_cast<std::size_t>(rp) * n;
       for (std::uint32_t i = tid; i < n; i += B) dst[i] = src[i];
     }
@@ -136,6 +167,7 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
   __syncthreads();
   for (std::uint32_t i = tid; i < npt; i += B) l.own[i] = l.work[i];
   __syncthreads();
+  st.mark(0);
 
   // CSC block of T: internal records [e0, ei), boundary records [ei, e1).
   const std::uint32_t e0 = g.column_offsets[v0];
@@ -144,8 +176,8 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
   const bool resident = ei - e0 <= static_cast<std::uint32_t>(B * EPT);
   if (tid == 0) {
     // edge records (pack + weight) once, the written column once
-    bytes += 8ull * (e1 - e0) + 4ull * npt;
-    edges += e1 - e0;
+    st.bytes += 8ull * (e1 - e0) + 4ull * npt;
+    st.edges += e1 - e0;
   }
   // Pass 1: every admitted in-edge reads the staged read column.
   auto relax_staged = [&](std::uint32_t p, float w) {
@@ -158,14 +190,16 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
     for (int j = 0; j < EPT; ++j)
       if (internal.pack[j] != kNone) relax_staged(internal.pack[j], internal.w[j]);
   }
-  for (std::uint32_t base = ei; base < e1; base += B * kStreamRecords) {
-    RecordRegs<B, kStreamRecords> chunk;
-    load_records<B, kStreamRecords>(chunk, g.csc_rec, base, e1);
+  st.mark(1);
+  for (std::uint32_t base = ei; base < e1; base += B * stream_records(B)) {
+    RecordRegs<B, stream_records(B)> chunk;
+    load_records<B, stream_records(B)>(chunk, g.csc_rec, base, e1);
 #pragma unroll
-    for (int j = 0; j < kStreamRecords; ++j)
+    for (int j = 0; j < stream_records(B); ++j)
       if (chunk.pack[j] != kNone) relax_staged(chunk.pack[j], chunk.w[j]);
   }
   __syncthreads();
+  st.mark(3);
   if (b.tile_sweeps == 0) {
     // Close the tile's internal edges in place; the internal records stay in
     // registers when they fit.
@@ -184,9 +218,11 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
           }
         }
       }
+      ++st.sweeps;
       if (!__syncthreads_or(changed)) break;
     }
   }
+  st.mark(2);
 
   std::uint32_t* out = b.dist + b.slot_dist[s] + static_cast<std::size_t>(rp ^ 1u) * npt;
   bool changed = false;
@@ -205,18 +241,18 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
       if (ns != kNone) enqueue(b, ns, r + 1);
     }
   }
+  st.mark(4);
 }
 
 // ---------------------------------------------------------------------------
 // Dense / frontier push visit.
 // ---------------------------------------------------------------------------
-template <int B, int EPT>
+template <int B, int EPT, bool I>
 __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b, std::uint32_t s,
                            std::uint32_t r, bool frontier, const Lds& l,
-                           RecordRegs<B, EPT>& internal, unsigned long long& attempts,
-                           unsigned long long& successes, unsigned long long& bytes,
-                           unsigned long long& edges) {
+                           RecordRegs<B, EPT>& internal, VisitStats<I>& st) {
   const std::uint32_t tid = threadIdx.x;
+  st.start();
   const std::uint32_t T = b.tiles[s];
   const std::uint32_t q = b.slot_query[s];
   const std::uint32_t v0 = g.tile_vertex_offsets[T];
@@ -227,9 +263,19 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
   std::uint32_t* column = b.dist + b.slot_dist[s];
   std::uint32_t* snapshot = column + npt;
 
+  const bool proposals = b.boundary_mode == 0;
   for (std::uint32_t k = tid; k < KO; k += B) {
-    l.table[k] = b.slot_out[out_base + k];
+    const std::uint32_t ns = b.slot_out[out_base + k];
+    l.table[k] = ns;
     l.table[256 + k] = 0;
+    if (proposals) {
+      l.table[512 + k] = g.out_nbr_stage[ob0 + k];
+    } else if (ns != kNone) {
+      // pool offset of the neighbour's labels, as two words
+      const unsigned long long off = b.slot_dist[ns];
+      l.table[512 + 2 * k] = static_cast<std::uint32_t>(off);
+      l.table[513 + 2 * k] = static_cast<std::uint32_t>(off >> 32);
+    }
   }
   for (std::uint32_t i = tid; i < npt; i += B) {
     const std::uint32_t x = load_agent(column + i);
@@ -238,7 +284,18 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
     // Dense pushes every finite vertex: an infinite snapshot admits them all.
     l.work[i] = frontier ? snapshot[i] : kInfBits;
   }
+  if (proposals) {
+    // Clear the proposal slab of every selected out-neighbour (k >= 1).
+    for (std::uint32_t k = 1; k < KO; ++k) {
+      if (b.slot_out[out_base + k] == kNone) continue;
+      const std::uint32_t nt = g.out_nbr_tiles[ob0 + k];
+      const std::uint32_t n = g.tile_vertex_offsets[nt + 1] - g.tile_vertex_offsets[nt];
+      std::uint32_t* slab = l.prop + g.out_nbr_stage[ob0 + k];
+      for (std::uint32_t i = tid; i < n; i += B) slab[i] = kInfBits;
+    }
+  }
   __syncthreads();
+  st.mark(0);
 
   // CSR block of T: internal records [e0, ei), boundary records [ei, e1).
   const std::uint32_t e0 = g.row_offsets[v0];
@@ -247,10 +304,11 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
   const bool resident = ei - e0 <= static_cast<std::uint32_t>(B * EPT);
   if (tid == 0) {
     // edge records once, the column (and the snapshot for frontier) once
-    bytes += 8ull * (e1 - e0) + 4ull * npt * (frontier ? 2 : 1);
-    edges += e1 - e0;
+    st.bytes += 8ull * (e1 - e0) + 4ull * npt * (frontier ? 2 : 1);
+    st.edges += e1 - e0;
   }
   if (resident) load_records<B, EPT>(internal, g.csr_rec, e0, ei);
+  st.mark(1);
   // Internal edges to the tile's local fixed point, from frontier sources.
   for (;;) {
     bool changed = false;
@@ -269,40 +327,97 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
         }
       }
     }
+    ++st.sweeps;
     if (!__syncthreads_or(changed)) break;
   }
-  // Boundary records, streamed once: push into neighbour slots with a global
-  // atomic min.  The read-first filter is an ordinary (cacheable) load: labels
-  // only fall during a run, so a stale copy is never below the current label
-  // and the filter never drops a decrease.
+  st.mark(2);
+  // Boundary records, streamed once.  Labels only fall during a run, so an
+  // ordinary (cacheable, possibly stale) load of a neighbour label is never
+  // below the current label: comparing against it never drops a decrease, at
+  // worst it issues an atomic or marks a neighbour that did not need it.
   unsigned long long att = 0, suc = 0;
-  for (std::uint32_t base = ei; base < e1; base += B * kStreamRecords) {
-    RecordRegs<B, kStreamRecords> chunk;
-    load_records<B, kStreamRecords>(chunk, g.csr_rec, base, e1);
+  if (proposals) {
+    // Gather the best candidate per destination in LDS, then compare each
+    // neighbour slab once, coalesced, and issue one atomic per improvement.
+    for (std::uint32_t base = ei; base < e1; base += B * stream_records(B)) {
+      RecordRegs<B, stream_records(B)> chunk;
+      load_records<B, stream_records(B)>(chunk, g.csr_rec, base, e1);
 #pragma unroll
-    for (int j = 0; j < kStreamRecords; ++j) {
-      const std::uint32_t p = chunk.pack[j];
-      if (p == kNone) continue;
-      const std::uint32_t sv = l.own[pk_src(p)];
-      if (!(sv < l.work[pk_src(p)])) continue;
-      const std::uint32_t ns = l.table[pk_k(p)];
+      for (int j = 0; j < stream_records(B); ++j) {
+        const std::uint32_t p = chunk.pack[j];
+        if (p == kNone) continue;
+        const std::uint32_t sv = l.own[pk_src(p)];
+        if (!(sv < l.work[pk_src(p)])) continue;
+        if (l.table[pk_k(p)] == kNone) continue;
+        const std::uint32_t cb = as_bits(as_float(sv) + chunk.w[j]);
+        std::uint32_t* slot = &l.prop[l.table[512 + pk_k(p)] + pk_dst(p)];
+        ++att;
+        if (cb < *slot) atomicMin(slot, cb);
+      }
+    }
+    __syncthreads();
+    for (std::uint32_t k = 1; k < KO; ++k) {
+      const std::uint32_t ns = l.table[k];
       if (ns == kNone) continue;
-      const std::uint32_t cb = as_bits(as_float(sv) + chunk.w[j]);
-      std::uint32_t* target = b.dist + b.slot_dist[ns] + pk_dst(p);
-      ++att;
-      if (cb < *target) {
-        const std::uint32_t old = atomic_min_agent(target, cb);
-        if (cb < old) {
+      const std::uint32_t nt = g.out_nbr_tiles[ob0 + k];
+      const std::uint32_t n = g.tile_vertex_offsets[nt + 1] - g.tile_vertex_offsets[nt];
+      const std::uint32_t* slab = l.prop + l.table[512 + k];
+      std::uint32_t* target = b.dist + b.slot_dist[ns];
+      if (tid == 0) st.bytes += 4ull * n;  // the neighbour slab, read once
+      bool hit = false;
+      for (std::uint32_t i = tid; i < n; i += B) {
+        const std::uint32_t pv = slab[i];
+        if (pv != kInfBits && pv < target[i]) {
+          atomic_min_agent(target + i, pv);
           ++suc;
-          l.table[256 + pk_k(p)] = 1;
+          hit = true;
+        }
+      }
+      if (hit) l.table[256 + k] = 1;
+    }
+  } else {
+    // One global atomic per boundary candidate that beats the neighbour's
+    // label as read.  Per chunk the reads are issued together, then the
+    // atomics, which return nothing: a neighbour is marked when the read
+    // label was above the candidate, which covers every actual decrease.
+    auto candidate = [&](std::uint32_t p, float w, std::uint32_t*& target) -> std::uint32_t {
+      if (p == kNone) return kNone;
+      const std::uint32_t sv = l.own[pk_src(p)];
+      if (!(sv < l.work[pk_src(p)]) || l.table[pk_k(p)] == kNone) return kNone;
+      const unsigned long long off = (static_cast<unsigned long long>(l.table[513 + 2 * pk_k(p)]) << 32) |
+                                     l.table[512 + 2 * pk_k(p)];
+      target = b.dist + off + pk_dst(p);
+      return as_bits(as_float(sv) + w);
+    };
+    for (std::uint32_t base = ei; base < e1; base += B * stream_records(B)) {
+      RecordRegs<B, stream_records(B)> chunk;
+      load_records<B, stream_records(B)>(chunk, g.csr_rec, base, e1);
+      std::uint32_t seen[stream_records(B)];
+#pragma unroll
+      for (int j = 0; j < stream_records(B); ++j) {
+        std::uint32_t* target = nullptr;
+        seen[j] = candidate(chunk.pack[j], chunk.w[j], target) == kNone ? 0u : *target;
+      }
+#pragma unroll
+      for (int j = 0; j < stream_records(B); ++j) {
+        std::uint32_t* target = nullptr;
+        const std::uint32_t cb = candidate(chunk.pack[j], chunk.w[j], target);
+        if (cb == kNone) continue;
+        ++att;
+        if (cb < seen[j]) {
+          atomic_min_agent(target, cb);
+          ++suc;
+          l.table[256 + pk_k(chunk.pack[j])] = 1;
         }
       }
     }
   }
-  attempts += att;
-  successes += suc;
-  // remote label read per attempt, read-modify-write per atomic issued
-  bytes += 4ull * att + 8ull * suc;
+  st.attempts += att;
+  st.successes += suc;
+  // direct: a remote label read per attempt; both: read-modify-write per
+  // atomic issued (proposal slabs read once, counted above)
+  st.bytes += (proposals ? 0ull : 4ull * att) + 8ull * suc;
+  st.mark(3);
   bool changed = false;
   for (std::uint32_t i = tid; i < npt; i += B) {
     const std::uint32_t v = l.own[i];
@@ -323,18 +438,19 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
   if (__syncthreads_or(marked) || pushed_any) {
     if (tid == 0) atomic_max_agent(&b.last_change[q], r + 1);
   }
+  st.mark(4);
 }
 
 // ---------------------------------------------------------------------------
 // Round processing shared by the persistent and host-polled kernels.
 // ---------------------------------------------------------------------------
-template <int B, int EPT>
+template <int B, int EPT, bool I>
 __device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch& b, std::uint32_t r,
                               std::uint32_t n, const Lds& l) {
   const std::uint32_t li = r % 3;
   const std::uint32_t chunk = b.items_per_claim ? b.items_per_claim : 1u;
   RecordRegs<B, EPT> regs;
-  unsigned long long attempts = 0, successes = 0, items = 0, bytes = 0, edges = 0;
+  VisitStats<I> st;
   for (;;) {
     if (threadIdx.x == 0) l.flags[0] = atomic_add_agent(&b.control->claim[li], chunk);
     __syncthreads();
@@ -346,24 +462,29 @@ __device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch&
       const std::uint32_t s = b.lists[static_cast<std::size_t>(li) * b.slot_count + i];
       const std::uint32_t engine = b.engine[b.slot_query[s]];
       if (engine == kJacobiPull) {
-        visit_pull<B, EPT>(g, b, s, r, l, regs, bytes, edges);
+        visit_pull<B, EPT, I>(g, b, s, r, l, regs, st);
       } else {
-        visit_push<B, EPT>(g, b, s, r, engine == kFrontierPush, l, regs, attempts, successes,
-                           bytes, edges);
+        visit_push<B, EPT, I>(g, b, s, r, engine == kFrontierPush, l, regs, st);
       }
-      ++items;
+      ++st.items;
       __syncthreads();
       // the visit's own enqueues are issued before its item is retired
       if (b.pending && threadIdx.x == 0) atomic_add_agent(&b.pending[b.slot_query[s]], 0xFFFFFFFFu);
     }
   }
-  if (b.instrumentation) {
+  if constexpr (I) {
     // One add per thread keeps the counters off the hot path's critical lanes.
-    if (attempts) atomicAdd(&b.control->atomic_attempts, attempts);
-    if (successes) atomicAdd(&b.control->atomic_successes, successes);
-    if (threadIdx.x == 0 && items) atomicAdd(&b.control->items_processed, items);
-    if (bytes) atomicAdd(&b.control->algorithmic_bytes, bytes);
-    if (edges) atomicAdd(&b.control->edges_relaxed, edges);
+    DevControl* c = b.control;
+    if (st.attempts) atomicAdd(&c->atomic_attempts, st.attempts);
+    if (st.successes) atomicAdd(&c->atomic_successes, st.successes);
+    if (threadIdx.x == 0) {
+      if (st.items) atomicAdd(&c->items_processed, st.items);
+      if (st.bytes) atomicAdd(&c->algorithmic_bytes, st.bytes);
+      if (st.edges) atomicAdd(&c->edges_relaxed, st.edges);
+      if (st.sweeps) atomicAdd(&c->closure_sweeps, st.sweeps);
+      for (int p = 0; p < kPhaseCount; ++p)
+        if (st.phase[p]) atomicAdd(&c->phase_cycles[p], st.phase[p]);
+    }
   }
 }
 
@@ -387,7 +508,7 @@ __device__ __forceinline__ void grid_barrier(DevControl* c, std::uint32_t blocks
 
 // One cooperative launch runs every round: process list r%3, grid barrier,
 // repeat while the next list is non-empty and the round limit allows.
-template <int B, int EPT>
+template <int B, int EPT, bool I>
 __global__ __launch_bounds__(B) void relax_persistent_kernel(DevGraph g, DevBatch b,
                                                              std::uint32_t max_v,
                                                              std::uint32_t work_words) {
@@ -405,7 +526,7 @@ __global__ __launch_bounds__(B) void relax_persistent_kernel(DevGraph g, DevBatc
       store_agent(&c->claim[(r + 2) % 3], 0u);
       if (b.instrumentation) atomicMax(&c->max_list, static_cast<unsigned long long>(n));
     }
-    process_round<B, EPT>(g, b, r, n, l);
+    process_round<B, EPT, I>(g, b, r, n, l);
     grid_barrier(c, gridDim.x);
     ++r;
   }
@@ -420,7 +541,7 @@ __global__ __launch_bounds__(B) void relax_persistent_kernel(DevGraph g, DevBatc
 // Host-polled control: one round per launch (ordinary grid), then the
 // one-thread advance (jacobi_pull.hpp:153-220 shape: count the round, reset
 // the list that the next round fills, stop on an empty list or the limit).
-template <int B, int EPT>
+template <int B, int EPT, bool I>
 __global__ __launch_bounds__(B) void relax_round_kernel(DevGraph g, DevBatch b,
                                                         std::uint32_t max_v,
                                                         std::uint32_t work_words) {
@@ -430,7 +551,7 @@ __global__ __launch_bounds__(B) void relax_round_kernel(DevGraph g, DevBatch b,
   if (load_agent(&c->done)) return;
   const std::uint32_t r = load_agent(&c->round);
   const std::uint32_t n = load_agent(&c->list_count[r % 3]);
-  process_round<B, EPT>(g, b, r, n, l);
+  process_round<B, EPT, I>(g, b, r, n, l);
 }
 
 __global__ void relax_advance_kernel(DevBatch b) {
@@ -475,6 +596,8 @@ __global__ void reset_control_kernel(DevControl* c) {
     c->max_list = 0;
     c->edges_relaxed = 0;
     c->algorithmic_bytes = 0;
+    c->closure_sweeps = 0;
+    for (int p = 0; p < kPhaseCount; ++p) c->phase_cycles[p] = 0;
   }
 }
 
@@ -540,20 +663,19 @@ __global__ void seed_kernel(DevGraph g, DevBatch b) {
 // ---------------------------------------------------------------------------
 // Launch wrappers (host side of this translation unit).
 // ---------------------------------------------------------------------------
-template <int B, int EPT>
+template <int B, int EPT, bool I>
 hipError_t launch_relax(bool persistent, bool cooperative, std::uint32_t grid,
                         std::size_t lds_bytes, std::uint32_t max_v, std::uint32_t work_words,
                         const DevGraph& g, const DevBatch& b, hipStream_t stream) {
   if (persistent) {
     void* args[] = {const_cast<DevGraph*>(&g), const_cast<DevBatch*>(&b), &max_v, &work_words};
+    const void* kernel = reinterpret_cast<const void*>(&relax_persistent_kernel<B, EPT, I>);
     if (cooperative)
-      return hipLaunchCooperativeKernel(
-          reinterpret_cast<const void*>(&relax_persistent_kernel<B, EPT>), dim3(grid), dim3(B),
-          args, static_cast<unsigned>(lds_bytes), stream);
-    return hipLaunchKernel(reinterpret_cast<const void*>(&relax_persistent_kernel<B, EPT>),
-                           dim3(grid), dim3(B), args, lds_bytes, stream);
+      return hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(B), args,
+                                        static_cast<unsigned>(lds_bytes), stream);
+    return hipLaunchKernel(kernel, dim3(grid), dim3(B), args, lds_bytes, stream);
   }
-  hipLaunchKernelGGL((relax_round_kernel<B, EPT>), dim3(grid), dim3(B), lds_bytes, stream, g, b,
+  hipLaunchKernelGGL((relax_round_kernel<B, EPT, I>), dim3(grid), dim3(B), lds_bytes, stream, g, b,
                      max_v, work_words);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -561,14 +683,12 @@ hipError_t launch_relax(bool persistent, bool cooperative, std::uint32_t grid,
   return hipGetLastError();
 }
 
-template <int B, int EPT>
+template <int B, int EPT, bool I>
 hipError_t occupancy_relax(bool persistent, std::size_t lds_bytes, int* blocks_per_cu) {
-  if (persistent)
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        blocks_per_cu, reinterpret_cast<const void*>(&relax_persistent_kernel<B, EPT>), B,
-        lds_bytes);
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      blocks_per_cu, reinterpret_cast<const void*>(&relax_round_kernel<B, EPT>), B, lds_bytes);
+  const void* kernel = persistent
+                           ? reinterpret_cast<const void*>(&relax_persistent_kernel<B, EPT, I>)
+                           : reinterpret_cast<const void*>(&relax_round_kernel<B, EPT, I>);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, kernel, B, lds_bytes);
 }
 
 // Block sizes the engine is instantiated for, each with the registers per
@@ -582,16 +702,22 @@ hipError_t relax_launch(std::uint32_t block, bool persistent, bool cooperative,
                         hipStream_t stream) {
 #define BFN_CASE(BS, E) \
   if (block == BS)      \
-    return launch_relax<BS, E>(persistent, cooperative, grid, lds_bytes, max_v, work_words, g, b, stream);
+    return b.instrumentation                                                                   \
+               ? launch_relax<BS, E, true>(persistent, cooperative, grid, lds_bytes, max_v,     \
+                                           work_words, g, b, stream)                           \
+               : launch_relax<BS, E, false>(persistent, cooperative, grid, lds_bytes, max_v,    \
+                                            work_words, g, b, stream);
   BFN_FOR_EACH_SHAPE(BFN_CASE)
 #undef BFN_CASE
   return hipErrorInvalidValue;
 }
 
-hipError_t relax_occupancy(std::uint32_t block, bool persistent, std::size_t lds_bytes,
-                           int* blocks_per_cu) {
-#define BFN_CASE(BS, E) \
-  if (block == BS) return occupancy_relax<BS, E>(persistent, lds_bytes, blocks_per_cu);
+hipError_t relax_occupancy(std::uint32_t block, bool persistent, bool instrumented,
+                           std::size_t lds_bytes, int* blocks_per_cu) {
+#define BFN_CASE(BS, E)                                                              \
+  if (block == BS)                                                                   \
+    return instrumented ? occupancy_relax<BS, E, true>(persistent, lds_bytes, blocks_per_cu) \
+                        : occupancy_relax<BS, E, false>(persistent, lds_bytes, blocks_per_cu);
   BFN_FOR_EACH_SHAPE(BFN_CASE)
 #undef BFN_CASE
   return hipErrorInvalidValue;

@@ -446,8 +446,10 @@ int32_t bfn_server_create(bfn_device device, bfn_graph graph, const bfn_server_o
     return bail(BFN_E_INVALID_ARGUMENT, "graph view unavailable");
   sv->max_v = sv->gview.max_tile_vertices;
   sv->slab_words = 2 * sv->max_v;
+  // Push engines push boundary edges with direct atomics here (no proposal
+  // area), which keeps more workgroups resident for small requests.
   sv->work_words = std::max(o->engine == kJacobiPull ? sv->gview.max_stage_floats : 0u, sv->max_v);
-  sv->lds_bytes = 4ull * (2ull * sv->max_v + sv->work_words + 512 + 8);
+  sv->lds_bytes = 4ull * (2ull * sv->max_v + sv->work_words + kLdsTableWords + kLdsFlagWords);
   if (sv->lds_bytes > device_lds_limit(device))
     return bail(BFN_E_NOT_REPRESENTABLE, "tile neighbourhood exceeds the LDS budget");
   sv->block = o->block_size;
@@ -502,6 +504,7 @@ int32_t bfn_server_create(bfn_device device, bfn_graph graph, const bfn_server_o
   b.tile_sweeps = o->tile_sweeps;
   b.items_per_claim = 1;
   b.instrumentation = 0;
+  b.boundary_mode = 1;
   b.slot_begin = sv->d_begin;
   b.slot_end = sv->d_end;
   b.tiles = sv->d_tiles;
@@ -527,7 +530,7 @@ int32_t bfn_server_create(bfn_device device, bfn_graph graph, const bfn_server_o
                      sv->Scap, sv->slab_words, sv->kin, sv->kout, sv->d_active, sv->d_src_off,
                      sv->d_tgt_off, sv->d_clean);
   int per_cu = 0;
-  if (relax_occupancy(sv->block, false, sv->lds_bytes, &per_cu) != hipSuccess || per_cu <= 0)
+  if (relax_occupancy(sv->block, false, false, sv->lds_bytes, &per_cu) != hipSuccess || per_cu <= 0)
     return bail(BFN_E_NOT_REPRESENTABLE, "relaxation kernel cannot be resident");
   sv->grid = static_cast<std::uint32_t>(per_cu) * device_cu_count(device);
   if (hipStreamSynchronize(sv->stream) != hipSuccess) return bail(BFN_E_HIP, "server init failed");

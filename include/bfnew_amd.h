@@ -130,7 +130,12 @@ typedef struct {
   uint32_t graph_rounds;         /* rounds captured per hipGraph replay */
   uint32_t use_graph;            /* replay the batch step through a hipGraph */
   uint32_t count_relaxations;    /* compute per-query relaxation counts */
-  uint32_t reserved[3];
+  uint32_t boundary_mode;        /* push engines: 0 = aggregate boundary pushes
+                                    per destination in LDS, then one coalesced
+                                    compare and an atomic per improved label
+                                    (when the LDS holds the proposals);
+                                    1 = one global atomic per boundary edge */
+  uint32_t reserved[2];
 } bfn_tuning;
 
 /* The reference DeviceGraphLayout32 (device_layout.hpp:72-93) plus the tile
@@ -288,7 +293,11 @@ typedef struct {
   uint64_t atomic_successes;
   uint64_t maximum_list;
   uint64_t rounds_executed;
-  uint64_t reserved;
+  uint64_t closure_sweeps;       /* tile-closure sweeps over internal records */
+  uint64_t phase_cycles[5];      /* shader-clock cycles per visit phase summed
+                                    over workgroups: setup and label staging,
+                                    internal records, tile closure, boundary
+                                    records, write-back and enqueue */
 } bfn_engine_counters;
 int32_t bfn_batch_counters(bfn_workspace workspace, bfn_engine_counters *out);
 

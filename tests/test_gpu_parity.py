@@ -17,9 +17,15 @@ CONTROLS = (bfn.PERSISTENT, bfn.CHUNKED, bfn.PER_ROUND)
 
 
 def configurations():
+    """(engine, control, knobs): Jacobi in point and block mode, the push
+    engines with LDS boundary proposals and with direct atomics."""
     for eng, ctl in itertools.product(ENGINES, CONTROLS):
-        for sweeps in ((1, 0) if eng == bfn.JACOBI_PULL else (1,)):
-            yield eng, ctl, sweeps
+        if eng == bfn.JACOBI_PULL:
+            for sweeps in (1, 0):
+                yield eng, ctl, {"sweeps": sweeps}
+        else:
+            for mode in (0, 1):
+                yield eng, ctl, {"boundary_mode": mode}
 
 
 def fixture_graph(device, case):
@@ -44,14 +50,14 @@ def test_fixture_suite_all_engines(device, golden):
         src = [int(view["old_to_new"][s]) for s in case["sources"]]
         tgt = [int(view["old_to_new"][t]) for t in case["targets"]]
         batch = bfn.Batch(sources=[src], targets=[tgt], tiles=[case["selected_tiles"]])
-        for eng, ctl, sweeps in configurations():
-            label = (case["name"], eng, ctl, sweeps)
-            ws = run(device, dg, batch, eng, ctl, sweeps)
+        for eng, ctl, knobs in configurations():
+            label = (case["name"], eng, ctl, knobs)
+            ws = run(device, dg, batch, eng, ctl, **knobs)
             st, per, _ = ws.status()
             assert st.converged == 1 and per[0].converged == 1, label
             d = ws.distances(0)
             assert list(bits(d)) == exp["distance_bits"], label
-            if eng == bfn.JACOBI_PULL and sweeps == 1:
+            if eng == bfn.JACOBI_PULL and knobs["sweeps"] == 1:
                 # point Jacobi keeps the reference's round count
                 assert per[0].rounds_completed == exp["jacobi_rounds"], label
                 assert st.rounds_completed == exp["jacobi_rounds"], label
@@ -116,15 +122,15 @@ def test_batched_mixed_duration_lanes(device, golden):
     batch = bfn.Batch(sources=[[int(view["old_to_new"][s]) for s in l["sources"]] for l in lanes],
                       targets=[[int(view["old_to_new"][t]) for t in l["targets"]] for l in lanes],
                       tiles=[l["selected_tiles"] for l in lanes])
-    for eng, ctl, sweeps in configurations():
-        ws = run(device, dg, batch, eng, ctl, sweeps)
+    for eng, ctl, knobs in configurations():
+        ws = run(device, dg, batch, eng, ctl, **knobs)
         st, per, _ = ws.status()
         for q, lane in enumerate(lanes):
             assert list(bits(ws.distances(q))) == lane["distance_bits"], (eng, ctl, q)
-            if eng == bfn.JACOBI_PULL and sweeps == 1:
+            if eng == bfn.JACOBI_PULL and knobs["sweeps"] == 1:
                 assert per[q].rounds_completed == lane["jacobi_rounds"], (ctl, q)
             assert per[q].reached_all_targets == lane["reached"]
-        if eng == bfn.JACOBI_PULL and sweeps == 1:
+        if eng == bfn.JACOBI_PULL and knobs["sweeps"] == 1:
             assert st.rounds_completed == max(l["jacobi_rounds"] for l in lanes)
         assert st.valid_lane_mask == 0b11111
         ws.close()
@@ -177,9 +183,9 @@ def test_c1_full_graph_distances_and_predecessors(device):
                                 view["in_weights"], view["in_edge_ids"], view["owner_tiles"], None,
                                 ref, [src])
     batch = bfn.Batch([[src]], [[tgt]], [list(range(T))])
-    for eng, ctl, sweeps in configurations():
-        ws = run(device, dg, batch, eng, ctl, sweeps)
-        assert np.array_equal(bits(ws.distances(0)), bits(ref)), (eng, ctl, sweeps)
+    for eng, ctl, knobs in configurations():
+        ws = run(device, dg, batch, eng, ctl, **knobs)
+        assert np.array_equal(bits(ws.distances(0)), bits(ref)), (eng, ctl, knobs)
         assert np.array_equal(ws.predecessors(0), pv)
         p = ws.paths()[0]
         st, ov, oe = oracle.reconstruct(view["vertex_count"], view["column_offsets"],
@@ -200,10 +206,12 @@ def test_c2_batched_and_plain_bit_exact(device):
     view = hg.view()
     qs, batch = c2_batch(hg, 24)
     refs = [oracle_query(view, [q.source], batch.tiles[i])[0] for i, q in enumerate(qs)]
-    for eng, sweeps in ((0, 0), (0, 1), (1, 1), (2, 1)):
-        ws = run(device, dg, batch, eng, bfn.PERSISTENT, sweeps)
+    for eng, knobs in ((0, {"sweeps": 0}), (0, {"sweeps": 1}), (1, {"boundary_mode": 0}),
+                       (1, {"boundary_mode": 1}), (2, {"boundary_mode": 0}),
+                       (2, {"boundary_mode": 1})):
+        ws = run(device, dg, batch, eng, bfn.PERSISTENT, **knobs)
         for i in range(len(qs)):
-            assert np.array_equal(bits(ws.distances(i)), bits(refs[i])), (eng, sweeps, i)
+            assert np.array_equal(bits(ws.distances(i)), bits(refs[i])), (eng, knobs, i)
         ws.close()
     # plain mode: one query per launch
     for i in range(4):
@@ -269,9 +277,17 @@ def test_c3_batch_certificate(device):
     ws = run(device, dg, batch, bfn.JACOBI_PULL, bfn.PERSISTENT, 0)
     st, per, _ = ws.status()
     assert st.converged == 1
-    for i in list(range(0, 256, 16)) + [i for i, q in enumerate(qs) if q.unbounded][:2]:
+    # The tuned configuration: frontier push, LDS boundary proposals,
+    # hipGraph replay of the rounds.
+    tuned = run(device, dg, batch, bfn.FRONTIER_PUSH, bfn.CHUNKED, 0, block=256,
+                items_per_claim=2, use_graph=1, graph_rounds=36, boundary_mode=0)
+    assert tuned.status()[0].converged == 1
+    sample = list(range(0, 256, 16)) + [i for i, q in enumerate(qs) if q.unbounded][:2]
+    for i in sample:
         d = ws.distances(i)
         assert certify_sssp(lay, [qs[i].source], tiles[i], d) == "", i
+        assert np.array_equal(bits(tuned.distances(i)), bits(d)), i
+    assert [p["distance"] for p in tuned.paths()] == [p["distance"] for p in ws.paths()]
     view = dict(vertex_count=lay["vertex_count"], tile_count=lay["tile_count"],
                 row_offsets=lay["csr_row_offsets"].astype(np.uint64),
                 destinations=lay["csr_destinations"], weights=lay["csr_weights"],
