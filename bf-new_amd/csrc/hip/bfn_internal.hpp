@@ -23,8 +23,11 @@
 //                       tile closure and stream the boundary part once
 //
 //   batch (one workspace; any number of queries)
-//     slot s = (query q, selected tile T); slots of q are q's selected tiles in
-//     sorted order, slots slot_begin[q] .. slot_end[q].
+//     slot s = (query q, selected tile T).  A batch numbers its slots
+//     tile-major (all slots of tile 0, then tile 1, ...; by query within a
+//     tile), so the labels of one tile's queries sit together in the pool.
+//     query_tiles[tile_begin[q] .. tile_end[q]) are q's selected tiles, sorted;
+//     query_slot[] holds the slot of each of those positions.
 //     slot_dist[s]   float offset of the slot's two npt-long copies in the pool:
 //                    Jacobi uses them as the two distance columns (read r&1,
 //                    write (r+1)&1); push engines keep distances in copy 0 and
@@ -129,8 +132,10 @@ struct DevBatch {
   std::uint32_t instrumentation;
   std::uint32_t boundary_mode;    // push: 0 LDS proposals when they fit, 1 direct atomics
   std::uint32_t pad;
-  const std::uint32_t* slot_begin;      // [Q]   first slot of q
-  const std::uint32_t* slot_end;        // [Q]   one past q's last slot
+  const std::uint32_t* tile_begin;      // [Q]   first position of q in query_tiles
+  const std::uint32_t* tile_end;        // [Q]   one past q's last position
+  const std::uint32_t* query_tiles;     // [.]   q's selected tiles, sorted
+  const std::uint32_t* query_slot;      // [.]   slot of each position, or null: the same
   const std::uint32_t* tiles;           // [S]   tile of slot
   const std::uint32_t* source_offsets;  // [Q+1]
   const std::uint32_t* sources;
@@ -211,16 +216,41 @@ __device__ __forceinline__ std::uint32_t atomic_add_agent(std::uint32_t* p, std:
 __device__ __forceinline__ float as_float(std::uint32_t b) { return __uint_as_float(b); }
 __device__ __forceinline__ std::uint32_t as_bits(float f) { return __float_as_uint(f); }
 
-// Binary search of tile in the sorted selected-tile list of a query.
-__device__ __forceinline__ std::uint32_t find_slot(const std::uint32_t* tiles,
-                                                   std::uint32_t begin, std::uint32_t end,
-                                                   std::uint32_t tile) {
+// Queues slot for `round` once: the stamp dedups, the list append claims a
+// position, and the query's pending work (when tracked) counts the item.
+__device__ __forceinline__ void enqueue(const DevBatch& b, std::uint32_t slot,
+                                        std::uint32_t round) {
+  const std::uint32_t tag = round + 1;
+  if (atomic_max_agent(&b.stamp[slot], tag) < tag) {
+    const std::uint32_t li = round % 3;
+    const std::uint32_t pos = atomic_add_agent(&b.control->list_count[li], 1u);
+    b.lists[static_cast<std::size_t>(li) * b.slot_count + pos] = slot;
+    if (b.pending) atomic_add_agent(&b.pending[b.slot_query[slot]], 1u);
+  }
+}
+
+// Position of key in the sorted range values[begin, end), or kNone.
+__device__ __forceinline__ std::uint32_t find_sorted(const std::uint32_t* values,
+                                                     std::uint32_t begin, std::uint32_t end,
+                                                     std::uint32_t key) {
   std::uint32_t lo = begin, hi = end;
   while (lo < hi) {
     std::uint32_t mid = (lo + hi) >> 1;
-    if (tiles[mid] < tile) lo = mid + 1; else hi = mid;
+    if (values[mid] < key) lo = mid + 1; else hi = mid;
   }
-  return (lo < end && tiles[lo] == tile) ? lo : kNone;
+  return (lo < end && values[lo] == key) ? lo : kNone;
+}
+
+// Slot at position pos of a query's selected tiles.
+__device__ __forceinline__ std::uint32_t slot_at(const DevBatch& b, std::uint32_t pos) {
+  return b.query_slot ? b.query_slot[pos] : pos;
+}
+
+// Slot of query q on tile, or kNone when q did not select the tile.
+__device__ __forceinline__ std::uint32_t find_slot(const DevBatch& b, std::uint32_t q,
+                                                   std::uint32_t tile) {
+  const std::uint32_t pos = find_sorted(b.query_tiles, b.tile_begin[q], b.tile_end[q], tile);
+  return pos == kNone ? kNone : slot_at(b, pos);
 }
 
 }  // namespace bfn

@@ -7,10 +7,12 @@
 //   chunked / per_round    -- rounds_per_chunk (round kernel + one-thread
 //                             advance) pairs, then one controller copy and
 //                             stream sync (batched_jacobi.hip.cpp:1540-1587);
-//   bfn_tuning.use_graph   -- the whole step (preparation, graph_rounds round
-//                             pairs, finalisation, status copies) captured once
-//                             into a hipGraph and replayed; rounds beyond
-//                             convergence exit at their first instruction.
+//   bfn_tuning.use_graph   -- graph_rounds round pairs and the controller
+//                             copy captured once into a hipGraph and replayed
+//                             after the preparation launches; rounds beyond
+//                             convergence exit at their first instruction,
+//                             and a host-polled tail finishes a batch that
+//                             needs more rounds than were captured.
 #include "bfnew_amd.h"
 
 #include "bfn_launch.hpp"
@@ -512,10 +514,10 @@ int32_t bfn_graph_update_node_costs(bfn_graph graph, const float* base_cost,
 int32_t bfn_graph_update_node_costs_from_workspace(bfn_graph graph, bfn_workspace ws,
                                                    const float* base_cost) {
   if (!graph || !ws) return fail(BFN_E_INVALID_ARGUMENT, "null argument");
-  if (!ws->occupancy.ptr || ws->graph != graph)
-    return fail(BFN_E_STATE, "workspace holds no occupancy for this graph");
+  if (ws->graph != graph) return fail(BFN_E_STATE, "workspace ran on another graph");
   BFN_HIP(hipSetDevice(graph->device->ordinal));
-  BFN_HIP(hipStreamSynchronize(ws->stream));
+  // Count the occupancy of the workspace's current compact paths on device.
+  if (int32_t rc = bfn_batch_occupancy(ws, nullptr)) return rc;
   return apply_node_costs(graph, base_cost, ws->occupancy.as<int>());
 }
 
@@ -582,7 +584,7 @@ int32_t validate_options(const bfn_run_options* o) {
 struct BatchPlanHost {
   std::uint32_t Q = 0, S = 0;
   std::uint64_t pool_words = 0, frame_words = 0, in_total = 0, out_total = 0;
-  std::vector<std::uint32_t> slot_query, slot_in_base, slot_out_base, engine;
+  std::vector<std::uint32_t> slot_query, slot_tile, query_slot, slot_in_base, slot_out_base, engine;
   std::vector<unsigned long long> slot_dist, frame_base;
   bool any_jacobi = false;
   bool any_push = false;
@@ -633,30 +635,44 @@ int32_t plan_batch(bfn_graph g, const bfn_batch_desc* d, const bfn_run_options* 
     return fail(BFN_E_INVALID_ARGUMENT, "offsets must start at zero");
   p.Q = Q;
   p.S = d->tile_offsets[Q];
-  p.slot_query.resize(p.S);
-  p.slot_in_base.resize(p.S);
-  p.slot_out_base.resize(p.S);
-  p.slot_dist.resize(p.S);
+  const std::uint32_t S = p.S;
+  p.slot_query.resize(S);
+  p.slot_tile.resize(S);
+  p.query_slot.resize(S);
+  p.slot_in_base.resize(S);
+  p.slot_out_base.resize(S);
+  p.slot_dist.resize(S);
   p.engine.resize(Q);
   p.frame_base.resize(Q);
-  std::uint64_t pool = 0, in_total = 0, out_total = 0, frames = 0;
+  // Tile-major slot numbering (counting sort of the (tile, query) pairs).
+  std::vector<std::uint32_t> cursor(g->T + 1, 0);
+  for (std::uint32_t i = 0; i < S; ++i) ++cursor[d->selected_tiles[i] + 1];
+  for (std::uint32_t t = 0; t < g->T; ++t) cursor[t + 1] += cursor[t];
+  std::uint64_t frames = 0;
   for (std::uint32_t q = 0; q < Q; ++q) {
     p.engine[q] = d->engine_by_query ? d->engine_by_query[q] : o->engine;
     p.any_jacobi |= p.engine[q] == kJacobiPull;
     p.any_push |= p.engine[q] != kJacobiPull;
     p.frame_base[q] = frames;
-    for (std::uint32_t s = d->tile_offsets[q]; s < d->tile_offsets[q + 1]; ++s) {
-      const std::uint32_t T = d->selected_tiles[s];
-      const std::uint32_t n = g->h_tile_off[T + 1] - g->h_tile_off[T];
+    for (std::uint32_t pos = d->tile_offsets[q]; pos < d->tile_offsets[q + 1]; ++pos) {
+      const std::uint32_t T = d->selected_tiles[pos];
+      const std::uint32_t s = cursor[T]++;
+      p.query_slot[pos] = s;
+      p.slot_tile[s] = T;
       p.slot_query[s] = q;
-      p.slot_dist[s] = pool;
-      pool += 2ull * n;
-      frames += n;
-      p.slot_in_base[s] = static_cast<std::uint32_t>(in_total);
-      p.slot_out_base[s] = static_cast<std::uint32_t>(out_total);
-      in_total += g->h_in_off[T + 1] - g->h_in_off[T];
-      out_total += g->h_out_off[T + 1] - g->h_out_off[T];
+      frames += g->h_tile_off[T + 1] - g->h_tile_off[T];
     }
+  }
+  // Labels and neighbour tables in slot order: a tile's queries sit together.
+  std::uint64_t pool = 0, in_total = 0, out_total = 0;
+  for (std::uint32_t s = 0; s < S; ++s) {
+    const std::uint32_t T = p.slot_tile[s];
+    p.slot_dist[s] = pool;
+    pool += 2ull * (g->h_tile_off[T + 1] - g->h_tile_off[T]);
+    p.slot_in_base[s] = static_cast<std::uint32_t>(in_total);
+    p.slot_out_base[s] = static_cast<std::uint32_t>(out_total);
+    in_total += g->h_in_off[T + 1] - g->h_in_off[T];
+    out_total += g->h_out_off[T + 1] - g->h_out_off[T];
   }
   if (in_total >= 0xFFFFFFFFull || out_total >= 0xFFFFFFFFull)
     return fail(BFN_E_NOT_REPRESENTABLE, "batch neighbour tables exceed 32-bit indexing");
@@ -667,9 +683,29 @@ int32_t plan_batch(bfn_graph g, const bfn_batch_desc* d, const bfn_run_options* 
   return BFN_OK;
 }
 
+// Batch metadata, uploaded in one copy: 0 tile offsets, 1 query tiles, 2-5
+// sources / targets, 6 engines, 7 slot query, 8 slot pool offset, 9-10
+// neighbour bases, 11 query slot, 12 slot tile, 13 frame base (u64 per query,
+// used by compact paths).  Returns the total bytes.
+constexpr int kMetaSegments = 14;
+std::size_t meta_layout(std::uint32_t Q, std::uint32_t S, std::uint32_t ns, std::uint32_t nt,
+                        std::size_t* seg, std::size_t* sizes) {
+  const std::size_t n[kMetaSegments] = {4ull * (Q + 1), 4ull * S, 4ull * (Q + 1), 4ull * ns,
+                                        4ull * (Q + 1), 4ull * nt, 4ull * Q, 4ull * S, 8ull * S,
+                                        4ull * S, 4ull * S, 4ull * S, 4ull * S, 8ull * Q};
+  std::size_t off = 0;
+  for (int i = 0; i < kMetaSegments; ++i) {
+    seg[i] = off;
+    sizes[i] = n[i];
+    off = align_up(off + n[i], 256);
+  }
+  return off;
+}
+
 std::uint64_t batch_bytes(const BatchPlanHost& p, std::uint64_t meta_bytes) {
-  return meta_bytes + 4ull * (p.in_total + p.out_total) + 4ull * p.S * 4 + 4ull * p.pool_words +
-         4ull * p.Q * 2 + sizeof(QueryStatusDev) * p.Q + 8ull * p.Q + sizeof(DevControl);
+  return meta_bytes + 4ull * (p.in_total + p.out_total) + 16ull * p.S +
+         4ull * p.pool_words + 4ull * p.Q * 2 + sizeof(QueryStatusDev) * p.Q + 8ull * p.Q +
+         sizeof(DevControl);
 }
 
 // Sets up the DevBatch view over the workspace buffers.
@@ -684,9 +720,11 @@ void bind_view(bfn_workspace ws, const BatchPlanHost& p, const std::size_t* seg,
   b.items_per_claim = std::max(1u, t->items_per_claim);
   b.instrumentation = o->instrumentation;
   b.boundary_mode = t->boundary_mode;
-  b.slot_begin = reinterpret_cast<const std::uint32_t*>(m + seg[0]);
-  b.slot_end = b.slot_begin + 1;
-  b.tiles = reinterpret_cast<const std::uint32_t*>(m + seg[1]);
+  b.tile_begin = reinterpret_cast<const std::uint32_t*>(m + seg[0]);
+  b.tile_end = b.tile_begin + 1;
+  b.query_tiles = reinterpret_cast<const std::uint32_t*>(m + seg[1]);
+  b.query_slot = reinterpret_cast<const std::uint32_t*>(m + seg[11]);
+  b.tiles = reinterpret_cast<const std::uint32_t*>(m + seg[12]);
   b.source_offsets = reinterpret_cast<const std::uint32_t*>(m + seg[2]);
   b.sources = reinterpret_cast<const std::uint32_t*>(m + seg[3]);
   b.target_offsets = reinterpret_cast<const std::uint32_t*>(m + seg[4]);
@@ -734,6 +772,9 @@ int32_t run_resident_batch(bfn_workspace ws) {
     return e;
   };
   if (graph_mode) {
+    // Preparation and finalisation run as ordinary launches; the graph holds
+    // the relaxation rounds and the control read-back, so the relax events
+    // bracket relaxation only.
     const std::uint32_t rounds = std::max(1u, t.graph_rounds);
     GraphKey key;
     key.block = ws->block;
@@ -751,11 +792,13 @@ int32_t run_resident_batch(bfn_workspace ws) {
       ws->graph_exec = nullptr;
       ws->graph_obj = nullptr;
       BFN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      hipError_t e = prep_launch(g->view, b, ws->pool_words, s);
+      hipError_t e = hipSuccess;
       for (std::uint32_t r = 0; r < rounds && e == hipSuccess; ++r)
         e = relax_launch(ws->block, false, false, ws->grid, ws->lds_bytes, ws->max_v,
                          ws->work_words, g->view, b, s);
-      if (e == hipSuccess) e = finalize();
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(ws->h_control.ptr, ws->control.ptr, sizeof(DevControl),
+                           hipMemcpyDeviceToHost, s);
       hipGraph_t captured = nullptr;
       const hipError_t end = hipStreamEndCapture(s, &captured);
       if (e != hipSuccess || end != hipSuccess)
@@ -765,9 +808,10 @@ int32_t run_resident_batch(bfn_workspace ws) {
       BFN_HIP(hipGraphInstantiate(&ws->graph_exec, captured, nullptr, nullptr, 0));
       ws->graph_key = key;
     }
+    BFN_HIP(prep_launch(g->view, b, ws->pool_words, s));
     BFN_HIP(hipEventRecord(ws->ev_relax0, s));
     BFN_HIP(hipGraphLaunch(ws->graph_exec, s));
-    ws->launches += 2ull * rounds + 4;
+    ws->launches += 2ull * rounds;
     BFN_HIP(hipStreamSynchronize(s));
     ++ws->host_syncs;
     ++ws->controller_copies;
@@ -785,9 +829,9 @@ int32_t run_resident_batch(bfn_workspace ws) {
       BFN_HIP(hipStreamSynchronize(s));
       ++ws->host_checks;
       ++ws->controller_copies;
-      if (hc->done) BFN_HIP(finalize());
     }
     BFN_HIP(hipEventRecord(ws->ev_relax1, s));
+    BFN_HIP(finalize());
     BFN_HIP(hipEventRecord(ws->ev_stop, s));
     return BFN_OK;
   }
@@ -829,9 +873,9 @@ int32_t bfn_batch_estimate(bfn_graph graph, const bfn_batch_desc* batch,
   if (int32_t rc = validate_options(options)) return rc;
   BatchPlanHost p;
   if (int32_t rc = plan_batch(graph, batch, options, p)) return rc;
-  const std::uint64_t meta = 4ull * (3ull * (p.Q + 1) + p.S * 4 + p.Q +
-                                     batch->source_offsets[p.Q] + batch->target_offsets[p.Q]) +
-                             8ull * (p.S + p.Q) + 1024;
+  std::size_t seg[kMetaSegments], sizes[kMetaSegments];
+  const std::size_t meta = meta_layout(p.Q, p.S, batch->source_offsets[p.Q],
+                                       batch->target_offsets[p.Q], seg, sizes);
   *bytes = batch_bytes(p, meta);
   return BFN_OK;
 }
@@ -854,17 +898,8 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
   const std::uint32_t Q = p.Q, S = p.S;
   const std::uint32_t ns = d->source_offsets[Q], nt = d->target_offsets[Q];
   // Metadata segments (one H2D copy from pinned staging).
-  std::size_t seg[12];
-  std::size_t off = 0;
-  const std::size_t sizes[11] = {4ull * (Q + 1), 4ull * S, 4ull * (Q + 1), 4ull * ns, 4ull * (Q + 1),
-                                 4ull * nt, 4ull * Q, 4ull * S, 8ull * S, 4ull * S, 4ull * S};
-  for (int i = 0; i < 11; ++i) {
-    seg[i] = off;
-    off = align_up(off + sizes[i], 256);
-  }
-  seg[11] = off;  // frame_base (u64 per query), used by compact paths
-  off = align_up(off + 8ull * Q, 256);
-  const std::size_t meta_bytes = off;
+  std::size_t seg[kMetaSegments], sizes[kMetaSegments];
+  const std::size_t meta_bytes = meta_layout(Q, S, ns, nt, seg, sizes);
   const std::uint64_t need = batch_bytes(p, meta_bytes);
   if (ws->budget && need > ws->budget)
     return fail(BFN_E_OUT_OF_MEMORY, "batch needs " + std::to_string(need) +
@@ -882,7 +917,9 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
   std::memcpy(h + seg[8], p.slot_dist.data(), sizes[8]);
   std::memcpy(h + seg[9], p.slot_in_base.data(), sizes[9]);
   std::memcpy(h + seg[10], p.slot_out_base.data(), sizes[10]);
-  std::memcpy(h + seg[11], p.frame_base.data(), 8ull * Q);
+  std::memcpy(h + seg[11], p.query_slot.data(), sizes[11]);
+  std::memcpy(h + seg[12], p.slot_tile.data(), sizes[12]);
+  std::memcpy(h + seg[13], p.frame_base.data(), sizes[13]);
 
   BFN_HIP(ws->meta.reserve(meta_bytes));
   BFN_HIP(ws->slot_in.reserve(4ull * std::max<std::uint64_t>(p.in_total, 1)));
@@ -910,7 +947,7 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
   ws->h_engine = p.engine;
   ws->launches = ws->host_checks = ws->host_syncs = ws->controller_copies = 0;
   bind_view(ws, p, seg, o, &t);
-  ws->frame_base_ptr = reinterpret_cast<const unsigned long long*>(ws->meta.as<char>() + seg[11]);
+  ws->frame_base_ptr = reinterpret_cast<const unsigned long long*>(ws->meta.as<char>() + seg[13]);
 
   // Launch shape.
   const DevGraph& gv = graph->view;

@@ -27,7 +27,7 @@ __device__ std::uint32_t label_of(const DevGraph& g, const DevBatch& b, std::uin
                                   std::uint32_t v, std::uint32_t jacobi_copy,
                                   unsigned long long* pool_index) {
   const std::uint32_t T = g.owner_tiles[v];
-  const std::uint32_t s = find_slot(b.tiles, b.slot_begin[q], b.slot_end[q], T);
+  const std::uint32_t s = find_slot(b, q, T);
   if (s == kNone) {
     if (pool_index) *pool_index = ~0ull;
     return kInfBits;
@@ -40,7 +40,7 @@ __device__ std::uint32_t label_of(const DevGraph& g, const DevBatch& b, std::uin
 }
 
 __device__ __forceinline__ bool is_source(const DevBatch& b, std::uint32_t q, std::uint32_t v) {
-  return find_slot(b.sources, b.source_offsets[q], b.source_offsets[q + 1], v) != kNone;
+  return find_sorted(b.sources, b.source_offsets[q], b.source_offsets[q + 1], v) != kNone;
 }
 
 __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long x) {
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(64) void compact_paths_kernel(PathArgs a) {
         break;
       }
       const std::uint32_t T = g.owner_tiles[v];
-      const std::uint32_t s = find_slot(b.tiles, b.slot_begin[q], b.slot_end[q], T);
+      const std::uint32_t s = find_slot(b, q, T);
       const std::uint32_t dv = label_of(g, b, q, v, copy, nullptr);
       const std::uint32_t in0 = g.in_nbr_offsets[T];
       unsigned long long best = ~0ull;
@@ -186,9 +186,9 @@ __global__ __launch_bounds__(64) void compact_paths_kernel(PathArgs a) {
 // Smallest-EdgeId tight admitted predecessor of each vertex of query q.
 __global__ void predecessors_kernel(DevGraph g, DevBatch b, std::uint32_t q,
                                     std::uint32_t jacobi_copy, std::uint32_t* out) {
-  const std::uint32_t first = b.slot_begin[q], last = b.slot_end[q];
-  const std::uint32_t s = first + blockIdx.x;
-  if (s >= last) return;
+  const std::uint32_t pos = b.tile_begin[q] + blockIdx.x;
+  if (pos >= b.tile_end[q]) return;
+  const std::uint32_t s = slot_at(b, pos);
   const std::uint32_t T = b.tiles[s];
   const std::uint32_t v0 = g.tile_vertex_offsets[T];
   const std::uint32_t npt = g.tile_vertex_offsets[T + 1] - v0;
@@ -223,8 +223,9 @@ __global__ void predecessors_kernel(DevGraph g, DevBatch b, std::uint32_t q,
 // Scatters query q's final labels to global ids (out pre-filled with +inf).
 __global__ void scatter_distances_kernel(DevGraph g, DevBatch b, std::uint32_t q,
                                          std::uint32_t jacobi_copy, std::uint32_t* out) {
-  const std::uint32_t s = b.slot_begin[q] + blockIdx.x;
-  if (s >= b.slot_end[q]) return;
+  const std::uint32_t pos = b.tile_begin[q] + blockIdx.x;
+  if (pos >= b.tile_end[q]) return;
+  const std::uint32_t s = slot_at(b, pos);
   const std::uint32_t T = b.tiles[s];
   const std::uint32_t v0 = g.tile_vertex_offsets[T];
   const std::uint32_t npt = g.tile_vertex_offsets[T + 1] - v0;

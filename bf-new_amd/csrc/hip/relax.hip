@@ -58,18 +58,6 @@ __device__ __forceinline__ Lds carve(std::uint32_t* base, std::uint32_t max_v,
   return l;
 }
 
-__device__ __forceinline__ void enqueue(const DevBatch& b, std::uint32_t slot,
-                                        std::uint32_t round) {
-  const std::uint32_t tag = round + 1;
-  const std::uint32_t old = atomic_max_agent(&b.stamp[slot], tag);
-  if (old < tag) {
-    const std::uint32_t li = round % 3;
-    const std::uint32_t pos = atomic_add_agent(&b.control->list_count[li], 1u);
-    b.lists[static_cast<std::size_t>(li) * b.slot_count + pos] = slot;
-    if (b.pending) atomic_add_agent(&b.pending[b.slot_query[slot]], 1u);
-  }
-}
-
 // Edge records in registers: thread t holds records t, t + B, t + 2B, ... of
 // the range it loaded.
 template <int B, int N>
@@ -136,7 +124,7 @@ __device__ __forceinline__ std::uint32_t pk_dst(std::uint32_t p) { return p & kP
 template <int B, int EPT, bool I>
 __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b, std::uint32_t s,
                            std::uint32_t r, const Lds& l, RecordRegs<B, EPT>& internal,
-                           VisitStats<I>& st) {
+                           std::uint32_t& regs_key, VisitStats<I>& st) {
   const std::uint32_t tid = threadIdx.x;
   st.start();
   const std::uint32_t T = b.tiles[s];
@@ -184,11 +172,20 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
     const std::uint32_t cb = as_bits(as_float(l.work[l.table[pk_k(p)] + pk_src(p)]) + w);
     if (cb < l.own[pk_dst(p)]) atomicMin(&l.own[pk_dst(p)], cb);
   };
-  for (std::uint32_t base = e0; base < ei; base += B * EPT) {
-    load_records<B, EPT>(internal, g.csc_rec, base, ei);
+  // The internal records stay in registers from a previous visit of T.
+  const std::uint32_t key = 2 * T;
+  if (resident && regs_key == key) {
 #pragma unroll
     for (int j = 0; j < EPT; ++j)
       if (internal.pack[j] != kNone) relax_staged(internal.pack[j], internal.w[j]);
+  } else {
+    for (std::uint32_t base = e0; base < ei; base += B * EPT) {
+      load_records<B, EPT>(internal, g.csc_rec, base, ei);
+#pragma unroll
+      for (int j = 0; j < EPT; ++j)
+        if (internal.pack[j] != kNone) relax_staged(internal.pack[j], internal.w[j]);
+    }
+    regs_key = resident ? key : kNone;
   }
   st.mark(1);
   for (std::uint32_t base = ei; base < e1; base += B * stream_records(B)) {
@@ -250,7 +247,8 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
 template <int B, int EPT, bool I>
 __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b, std::uint32_t s,
                            std::uint32_t r, bool frontier, const Lds& l,
-                           RecordRegs<B, EPT>& internal, VisitStats<I>& st) {
+                           RecordRegs<B, EPT>& internal, std::uint32_t& regs_key,
+                           VisitStats<I>& st) {
   const std::uint32_t tid = threadIdx.x;
   st.start();
   const std::uint32_t T = b.tiles[s];
@@ -307,7 +305,10 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
     st.bytes += 8ull * (e1 - e0) + 4ull * npt * (frontier ? 2 : 1);
     st.edges += e1 - e0;
   }
-  if (resident) load_records<B, EPT>(internal, g.csr_rec, e0, ei);
+  // The internal records stay in registers from a previous visit of T.
+  const std::uint32_t key = 2 * T + 1;
+  if (resident && regs_key != key) load_records<B, EPT>(internal, g.csr_rec, e0, ei);
+  regs_key = resident ? key : kNone;
   st.mark(1);
   // Internal edges to the tile's local fixed point, from frontier sources.
   for (;;) {
@@ -450,6 +451,7 @@ __device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch&
   const std::uint32_t li = r % 3;
   const std::uint32_t chunk = b.items_per_claim ? b.items_per_claim : 1u;
   RecordRegs<B, EPT> regs;
+  std::uint32_t regs_key = kNone;  // tile and orientation of the records in regs
   VisitStats<I> st;
   for (;;) {
     if (threadIdx.x == 0) l.flags[0] = atomic_add_agent(&b.control->claim[li], chunk);
@@ -462,9 +464,9 @@ __device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch&
       const std::uint32_t s = b.lists[static_cast<std::size_t>(li) * b.slot_count + i];
       const std::uint32_t engine = b.engine[b.slot_query[s]];
       if (engine == kJacobiPull) {
-        visit_pull<B, EPT, I>(g, b, s, r, l, regs, st);
+        visit_pull<B, EPT, I>(g, b, s, r, l, regs, regs_key, st);
       } else {
-        visit_push<B, EPT, I>(g, b, s, r, engine == kFrontierPush, l, regs, st);
+        visit_push<B, EPT, I>(g, b, s, r, engine == kFrontierPush, l, regs, regs_key, st);
       }
       ++st.items;
       __syncthreads();
@@ -615,15 +617,12 @@ __global__ void prep_slots_kernel(DevGraph g, DevBatch b) {
   if (s >= b.slot_count) return;
   const std::uint32_t q = b.slot_query[s];
   const std::uint32_t T = b.tiles[s];
-  const std::uint32_t qb = b.slot_begin[q], qe = b.slot_end[q];
   const std::uint32_t in0 = g.in_nbr_offsets[T], in1 = g.in_nbr_offsets[T + 1];
   const std::uint32_t ib = b.slot_in_base[s];
-  for (std::uint32_t k = in0; k < in1; ++k)
-    b.slot_in[ib + (k - in0)] = find_slot(b.tiles, qb, qe, g.in_nbr_tiles[k]);
+  for (std::uint32_t k = in0; k < in1; ++k) b.slot_in[ib + (k - in0)] = find_slot(b, q, g.in_nbr_tiles[k]);
   const std::uint32_t o0 = g.out_nbr_offsets[T], o1 = g.out_nbr_offsets[T + 1];
   const std::uint32_t obs = b.slot_out_base[s];
-  for (std::uint32_t k = o0; k < o1; ++k)
-    b.slot_out[obs + (k - o0)] = find_slot(b.tiles, qb, qe, g.out_nbr_tiles[k]);
+  for (std::uint32_t k = o0; k < o1; ++k) b.slot_out[obs + (k - o0)] = find_slot(b, q, g.out_nbr_tiles[k]);
   b.stamp[s] = 0;
   b.last_change[q] = 0;
 }
@@ -633,12 +632,11 @@ __global__ void prep_slots_kernel(DevGraph g, DevBatch b) {
 __global__ void seed_kernel(DevGraph g, DevBatch b) {
   const std::uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= b.query_count) return;
-  const std::uint32_t qb = b.slot_begin[q], qe = b.slot_end[q];
   const bool jacobi = b.engine[q] == kJacobiPull;
   for (std::uint32_t i = b.source_offsets[q]; i < b.source_offsets[q + 1]; ++i) {
     const std::uint32_t u = b.sources[i];
     const std::uint32_t T = g.owner_tiles[u];
-    const std::uint32_t s = find_slot(b.tiles, qb, qe, T);
+    const std::uint32_t s = find_slot(b, q, T);
     if (s == kNone) {
       atomicOr(&b.control->error_bits, 4u);  // device_failure: host validated this
       continue;

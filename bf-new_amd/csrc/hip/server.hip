@@ -62,8 +62,8 @@ __global__ void admit_kernel(DevGraph g, DevBatch b, const AdmitRecord* records,
     b.stamp[s] = 0;
   }
   if (tid == 0) {
-    const_cast<std::uint32_t*>(b.slot_begin)[q] = r.slot_begin;
-    const_cast<std::uint32_t*>(b.slot_end)[q] = r.slot_begin + r.slot_count;
+    const_cast<std::uint32_t*>(b.tile_begin)[q] = r.slot_begin;
+    const_cast<std::uint32_t*>(b.tile_end)[q] = r.slot_begin + r.slot_count;
     const_cast<std::uint32_t*>(b.sources)[q] = r.source;
     const_cast<std::uint32_t*>(b.targets)[q] = r.target;
     const_cast<std::uint32_t*>(b.engine)[q] = r.engine;
@@ -79,10 +79,10 @@ __global__ void admit_kernel(DevGraph g, DevBatch b, const AdmitRecord* records,
     const std::uint32_t T = b.tiles[s];
     const std::uint32_t in0 = g.in_nbr_offsets[T], in1 = g.in_nbr_offsets[T + 1];
     for (std::uint32_t k = in0; k < in1; ++k)
-      b.slot_in[static_cast<std::size_t>(s) * kin + (k - in0)] = find_slot(b.tiles, qb, qe, g.in_nbr_tiles[k]);
+      b.slot_in[static_cast<std::size_t>(s) * kin + (k - in0)] = find_slot(b, q, g.in_nbr_tiles[k]);
     const std::uint32_t o0 = g.out_nbr_offsets[T], o1 = g.out_nbr_offsets[T + 1];
     for (std::uint32_t k = o0; k < o1; ++k)
-      b.slot_out[static_cast<std::size_t>(s) * kout + (k - o0)] = find_slot(b.tiles, qb, qe, g.out_nbr_tiles[k]);
+      b.slot_out[static_cast<std::size_t>(s) * kout + (k - o0)] = find_slot(b, q, g.out_nbr_tiles[k]);
   }
   const unsigned long long w0 = static_cast<unsigned long long>(qb) * slab_words;
   const unsigned long long w1 = static_cast<unsigned long long>(qe) * slab_words;
@@ -96,7 +96,7 @@ __global__ void admit_kernel(DevGraph g, DevBatch b, const AdmitRecord* records,
     const std::uint32_t round = load_agent(&b.control->round);
     const std::uint32_t u = r.source;
     const std::uint32_t T = g.owner_tiles[u];
-    const std::uint32_t s = find_slot(b.tiles, qb, qe, T);
+    const std::uint32_t s = find_slot(b, q, T);
     if (s == kNone) {
       atomicOr(&b.control->error_bits, 4u);
       return;
@@ -107,22 +107,12 @@ __global__ void admit_kernel(DevGraph g, DevBatch b, const AdmitRecord* records,
     column[u - v0] = 0u;
     const bool jacobi = r.engine == kJacobiPull;
     if (jacobi) column[npt + (u - v0)] = 0u;
-    // enqueue() of relax.hip, with the pending count of q
-    auto put = [&](std::uint32_t slot) {
-      const std::uint32_t tag = round + 1;
-      if (atomic_max_agent(&b.stamp[slot], tag) < tag) {
-        const std::uint32_t li = round % 3;
-        const std::uint32_t pos = atomic_add_agent(&b.control->list_count[li], 1u);
-        b.lists[static_cast<std::size_t>(li) * b.slot_count + pos] = slot;
-        atomic_add_agent(&b.pending[q], 1u);
-      }
-    };
-    put(s);
+    enqueue(b, s, round);
     if (jacobi) {
       const std::uint32_t KO = g.out_nbr_offsets[T + 1] - g.out_nbr_offsets[T];
       for (std::uint32_t k = 0; k < KO; ++k) {
         const std::uint32_t ns = b.slot_out[static_cast<std::size_t>(s) * kout + k];
-        if (ns != kNone) put(ns);
+        if (ns != kNone) enqueue(b, ns, round);
       }
     }
     store_agent(&b.control->done, 0u);
@@ -141,7 +131,7 @@ __global__ void harvest_kernel(DevGraph g, DevBatch b, std::uint32_t capacity,
   done_list[1 + pos] = q;
   const std::uint32_t t = b.targets[q];
   const std::uint32_t T = g.owner_tiles[t];
-  const std::uint32_t s = find_slot(b.tiles, b.slot_begin[q], b.slot_end[q], T);
+  const std::uint32_t s = find_slot(b, q, T);
   std::uint32_t label = kInfBits;
   if (s != kNone) {
     const std::uint32_t v0 = g.tile_vertex_offsets[T];
@@ -505,8 +495,12 @@ int32_t bfn_server_create(bfn_device device, bfn_graph graph, const bfn_server_o
   b.items_per_claim = 1;
   b.instrumentation = 0;
   b.boundary_mode = 1;
-  b.slot_begin = sv->d_begin;
-  b.slot_end = sv->d_end;
+  // A request's slots form one contiguous range of the arena, in tile
+  // order, so positions and slots coincide.
+  b.tile_begin = sv->d_begin;
+  b.tile_end = sv->d_end;
+  b.query_tiles = sv->d_tiles;
+  b.query_slot = nullptr;
   b.tiles = sv->d_tiles;
   b.source_offsets = sv->d_src_off;
   b.sources = sv->d_src;

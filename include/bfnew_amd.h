@@ -232,7 +232,9 @@ int32_t bfn_graph_update_weights(bfn_graph graph, const float *csr_weights,
  * occupancy counts (bfn_batch_occupancy writes them on device). */
 int32_t bfn_graph_update_node_costs(bfn_graph graph, const float *base_cost,
                                     const int32_t *occupancy);
-/* Device-resident variant used between negotiated-cost rounds. */
+/* Device-resident variant used between negotiated-cost rounds: counts the
+ * occupancy of the workspace's last compact paths (bfn_batch_paths) on device
+ * and applies it (BFN_E_STATE when the workspace holds no paths). */
 int32_t bfn_graph_update_node_costs_from_workspace(bfn_graph graph,
                                                    bfn_workspace workspace,
                                                    const float *base_cost);
