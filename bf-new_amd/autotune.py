@@ -66,7 +66,8 @@ def time_candidate(dev, dg, batch, cand, reps):
         tune = bfn.tuning(tile_sweeps=cand["sweeps"], items_per_claim=cand["items_per_claim"],
                           use_graph=cand.get("use_graph", 0),
                           graph_rounds=cand.get("graph_rounds", 0),
-                          boundary_mode=cand.get("boundary_mode", 0))
+                          boundary_mode=cand.get("boundary_mode", 0),
+                          min_waves=cand.get("min_waves", 0))
         ws.submit(dg, batch, opts, tune)
         st, _, _ = ws.status()
         if not st.converged:
@@ -90,11 +91,14 @@ def candidates(engines, blocks, quick):
     out = []
     for engine, block in itertools.product(engines, blocks):
         jacobi = engine == bfn.JACOBI_PULL
+        waves = (0, 6) if (not jacobi and block in (256, 512)) else (0,)
         for sw in ((0, 1) if jacobi else (0,)):
             for mode in ((0,) if jacobi else (0, 1)):
-                for ipc in ((1,) if quick else (1, 2)):
-                    out.append(dict(engine=engine, block=block, sweeps=sw, items_per_claim=ipc,
-                                    boundary_mode=mode, control=bfn.PERSISTENT))
+                for mw in waves:
+                    for ipc in ((2,) if quick else (1, 2, 4)):
+                        out.append(dict(engine=engine, block=block, sweeps=sw, items_per_claim=ipc,
+                                        boundary_mode=mode, min_waves=mw,
+                                        control=bfn.PERSISTENT))
     return out
 
 
@@ -133,7 +137,8 @@ def main():
             raise SystemExit(f"candidate {cand} changed target distances")
         trials.append(dict(cand, ms=ms, rounds=rounds, scope="all"))
         log(f"{bfn.ENGINE_NAMES[cand['engine']]:>20} block {cand['block']:4d} sweeps "
-            f"{cand['sweeps']} boundary {cand['boundary_mode']} ipc {cand['items_per_claim']} control "
+            f"{cand['sweeps']} boundary {cand['boundary_mode']} waves {cand['min_waves']} "
+            f"ipc {cand['items_per_claim']} control "
             f"{bfn.CONTROL_NAMES[cand['control']]}{'+graph' if cand.get('use_graph') else ''}: "
             f"{ms:.3f} ms, {rounds} rounds")
         if cand["control"] == bfn.PERSISTENT and not args.quick:
@@ -155,7 +160,8 @@ def main():
         timing = {}
         for engine in engines:
             cand = dict(best, engine=engine, sweeps=best["sweeps"] if engine == 0 else 0,
-                        boundary_mode=0 if engine == 0 else best.get("boundary_mode", 0))
+                        boundary_mode=0 if engine == 0 else best.get("boundary_mode", 0),
+                        min_waves=0 if engine == 0 else best.get("min_waves", 0))
             ms, _, rounds = time_candidate(dev, dg, sub, cand, args.reps)
             if ms is not None:
                 timing[engine] = ms
@@ -164,7 +170,7 @@ def main():
         log(f"class {cls} ({len(idx)} queries):",
             {bfn.ENGINE_NAMES[e]: round(v, 3) for e, v in timing.items()})
     entry = {k: best[k] for k in ("engine", "block", "sweeps", "control", "items_per_claim",
-                                  "boundary_mode")}
+                                  "boundary_mode", "min_waves")}
     entry["use_graph"] = best.get("use_graph", 0)
     entry["graph_rounds"] = best.get("graph_rounds", 0)
     entry["ms"] = best["ms"]
@@ -191,8 +197,8 @@ def main():
     with open(table_path, "w") as f:
         json.dump(table, f, indent=1, sort_keys=True)
     with open(os.path.join(args.out, f"trials_{args.workload}.tsv"), "w") as f:
-        cols = ["scope", "engine", "block", "sweeps", "boundary_mode", "items_per_claim", "control",
-                "use_graph", "graph_rounds", "rounds", "ms"]
+        cols = ["scope", "engine", "block", "sweeps", "boundary_mode", "min_waves", "items_per_claim",
+                "control", "use_graph", "graph_rounds", "rounds", "ms"]
         f.write("\t".join(cols) + "\n")
         for t in trials:
             f.write("\t".join(str(t.get(c, 0)) for c in cols) + "\n")

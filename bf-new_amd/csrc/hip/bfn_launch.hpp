@@ -24,7 +24,7 @@ hipError_t relax_launch(std::uint32_t block, bool persistent, bool cooperative,
                         std::uint32_t work_words, const DevGraph& g, const DevBatch& b,
                         hipStream_t stream);
 hipError_t relax_occupancy(std::uint32_t block, bool persistent, bool instrumented,
-                           std::size_t lds_bytes, int* blocks_per_cu);
+                           std::uint32_t min_waves, std::size_t lds_bytes, int* blocks_per_cu);
 hipError_t prep_launch(const DevGraph& g, const DevBatch& b, std::uint64_t pool_words,
                        hipStream_t stream);
 // Inputs and outputs of the tile-image derivation (graph.hip).

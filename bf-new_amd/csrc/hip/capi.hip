@@ -720,6 +720,7 @@ void bind_view(bfn_workspace ws, const BatchPlanHost& p, const std::size_t* seg,
   b.items_per_claim = std::max(1u, t->items_per_claim);
   b.instrumentation = o->instrumentation;
   b.boundary_mode = t->boundary_mode;
+  b.min_waves = t->min_waves;
   b.tile_begin = reinterpret_cast<const std::uint32_t*>(m + seg[0]);
   b.tile_end = b.tile_begin + 1;
   b.query_tiles = reinterpret_cast<const std::uint32_t*>(m + seg[1]);
@@ -887,8 +888,9 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
   if (int32_t rc = validate_options(o)) return rc;
   bfn_tuning t;
   if (tuning) t = *tuning; else bfn_tuning_default(&t);
-  if (t.tile_sweeps > 1 || t.boundary_mode > 1 || t.use_graph > 1)
-    return fail(BFN_E_INVALID_ARGUMENT, "invalid tuning (tile_sweeps, boundary_mode, use_graph)");
+  if (t.tile_sweeps > 1 || t.boundary_mode > 1 || t.use_graph > 1 || t.min_waves > 8)
+    return fail(BFN_E_INVALID_ARGUMENT,
+                "invalid tuning (tile_sweeps, boundary_mode, use_graph, min_waves)");
   BatchPlanHost p;
   if (int32_t rc = plan_batch(graph, d, o, p)) return rc;
   BFN_HIP(hipSetDevice(ws->device->ordinal));
@@ -968,8 +970,8 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
                                              std::to_string(ws->lds_bytes) + " > " +
                                              std::to_string(lds_limit) + " bytes)");
   int per_cu = 0;
-  BFN_HIP(relax_occupancy(ws->block, o->control_mode == 0, o->instrumentation != 0, ws->lds_bytes,
-                          &per_cu));
+  BFN_HIP(relax_occupancy(ws->block, o->control_mode == 0, o->instrumentation != 0, t.min_waves,
+                          ws->lds_bytes, &per_cu));
   if (per_cu <= 0) return fail(BFN_E_NOT_REPRESENTABLE, "relaxation kernel cannot be resident");
   const std::uint32_t cus = static_cast<std::uint32_t>(ws->device->props.multiProcessorCount);
   std::uint32_t blocks = static_cast<std::uint32_t>(per_cu) * cus;

@@ -510,8 +510,8 @@ __device__ __forceinline__ void grid_barrier(DevControl* c, std::uint32_t blocks
 
 // One cooperative launch runs every round: process list r%3, grid barrier,
 // repeat while the next list is non-empty and the round limit allows.
-template <int B, int EPT, bool I>
-__global__ __launch_bounds__(B) void relax_persistent_kernel(DevGraph g, DevBatch b,
+template <int B, int EPT, bool I, int MW>
+__global__ __launch_bounds__(B, MW) void relax_persistent_kernel(DevGraph g, DevBatch b,
                                                              std::uint32_t max_v,
                                                              std::uint32_t work_words) {
   extern __shared__ std::uint32_t lds_words[];
@@ -543,8 +543,8 @@ __global__ __launch_bounds__(B) void relax_persistent_kernel(DevGraph g, DevBatc
 // Host-polled control: one round per launch (ordinary grid), then the
 // one-thread advance (jacobi_pull.hpp:153-220 shape: count the round, reset
 // the list that the next round fills, stop on an empty list or the limit).
-template <int B, int EPT, bool I>
-__global__ __launch_bounds__(B) void relax_round_kernel(DevGraph g, DevBatch b,
+template <int B, int EPT, bool I, int MW>
+__global__ __launch_bounds__(B, MW) void relax_round_kernel(DevGraph g, DevBatch b,
                                                         std::uint32_t max_v,
                                                         std::uint32_t work_words) {
   extern __shared__ std::uint32_t lds_words[];
@@ -661,64 +661,78 @@ __global__ void seed_kernel(DevGraph g, DevBatch b) {
 // ---------------------------------------------------------------------------
 // Launch wrappers (host side of this translation unit).
 // ---------------------------------------------------------------------------
-template <int B, int EPT, bool I>
+template <int B, int EPT, bool I, int MW>
 hipError_t launch_relax(bool persistent, bool cooperative, std::uint32_t grid,
                         std::size_t lds_bytes, std::uint32_t max_v, std::uint32_t work_words,
                         const DevGraph& g, const DevBatch& b, hipStream_t stream) {
   if (persistent) {
     void* args[] = {const_cast<DevGraph*>(&g), const_cast<DevBatch*>(&b), &max_v, &work_words};
-    const void* kernel = reinterpret_cast<const void*>(&relax_persistent_kernel<B, EPT, I>);
+    const void* kernel = reinterpret_cast<const void*>(&relax_persistent_kernel<B, EPT, I, MW>);
     if (cooperative)
       return hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(B), args,
                                         static_cast<unsigned>(lds_bytes), stream);
     return hipLaunchKernel(kernel, dim3(grid), dim3(B), args, lds_bytes, stream);
   }
-  hipLaunchKernelGGL((relax_round_kernel<B, EPT, I>), dim3(grid), dim3(B), lds_bytes, stream, g, b,
-                     max_v, work_words);
+  hipLaunchKernelGGL((relax_round_kernel<B, EPT, I, MW>), dim3(grid), dim3(B), lds_bytes, stream,
+                     g, b, max_v, work_words);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(relax_advance_kernel, dim3(1), dim3(1), 0, stream, b);
   return hipGetLastError();
 }
 
-template <int B, int EPT, bool I>
+template <int B, int EPT, bool I, int MW>
 hipError_t occupancy_relax(bool persistent, std::size_t lds_bytes, int* blocks_per_cu) {
-  const void* kernel = persistent
-                           ? reinterpret_cast<const void*>(&relax_persistent_kernel<B, EPT, I>)
-                           : reinterpret_cast<const void*>(&relax_round_kernel<B, EPT, I>);
+  const void* kernel =
+      persistent ? reinterpret_cast<const void*>(&relax_persistent_kernel<B, EPT, I, MW>)
+                 : reinterpret_cast<const void*>(&relax_round_kernel<B, EPT, I, MW>);
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, kernel, B, lds_bytes);
 }
 
-// Block sizes the engine is instantiated for, each with the registers per
+// Instantiated kernel variants.  Block sizes come with the registers per
 // thread that hold a tile's internal records (B * EPT = 2048: 512-vertex
 // tiles with four internal out-edges per vertex stay register-resident).
-#define BFN_FOR_EACH_SHAPE(X) X(64, 32) X(128, 16) X(256, 8) X(512, 4) X(1024, 2)
+// The 256- and 512-thread blocks also come bounded to at least 6 or 8 waves
+// per SIMD (fewer registers per lane, more visits in flight); instrumented
+// runs use the unbounded variant.
+template <int B, int EPT, typename F>
+hipError_t with_waves(bool instrumented, std::uint32_t min_waves, F& f) {
+  if (instrumented) return f.template operator()<B, EPT, true, 1>();
+  if constexpr (B == 256 || B == 512) {
+    if (min_waves >= 8) return f.template operator()<B, EPT, false, 8>();
+    if (min_waves >= 6) return f.template operator()<B, EPT, false, 6>();
+  }
+  return f.template operator()<B, EPT, false, 1>();
+}
+
+template <typename F>
+hipError_t with_shape(std::uint32_t block, bool instrumented, std::uint32_t min_waves, F&& f) {
+  switch (block) {
+    case 64: return with_waves<64, 32>(instrumented, min_waves, f);
+    case 128: return with_waves<128, 16>(instrumented, min_waves, f);
+    case 256: return with_waves<256, 8>(instrumented, min_waves, f);
+    case 512: return with_waves<512, 4>(instrumented, min_waves, f);
+    case 1024: return with_waves<1024, 2>(instrumented, min_waves, f);
+    default: return hipErrorInvalidValue;
+  }
+}
 
 hipError_t relax_launch(std::uint32_t block, bool persistent, bool cooperative,
                         std::uint32_t grid, std::size_t lds_bytes, std::uint32_t max_v,
                         std::uint32_t work_words, const DevGraph& g, const DevBatch& b,
                         hipStream_t stream) {
-#define BFN_CASE(BS, E) \
-  if (block == BS)      \
-    return b.instrumentation                                                                   \
-               ? launch_relax<BS, E, true>(persistent, cooperative, grid, lds_bytes, max_v,     \
-                                           work_words, g, b, stream)                           \
-               : launch_relax<BS, E, false>(persistent, cooperative, grid, lds_bytes, max_v,    \
-                                            work_words, g, b, stream);
-  BFN_FOR_EACH_SHAPE(BFN_CASE)
-#undef BFN_CASE
-  return hipErrorInvalidValue;
+  return with_shape(block, b.instrumentation != 0, b.min_waves,
+                    [&]<int B, int EPT, bool I, int MW>() {
+                      return launch_relax<B, EPT, I, MW>(persistent, cooperative, grid, lds_bytes,
+                                                         max_v, work_words, g, b, stream);
+                    });
 }
 
 hipError_t relax_occupancy(std::uint32_t block, bool persistent, bool instrumented,
-                           std::size_t lds_bytes, int* blocks_per_cu) {
-#define BFN_CASE(BS, E)                                                              \
-  if (block == BS)                                                                   \
-    return instrumented ? occupancy_relax<BS, E, true>(persistent, lds_bytes, blocks_per_cu) \
-                        : occupancy_relax<BS, E, false>(persistent, lds_bytes, blocks_per_cu);
-  BFN_FOR_EACH_SHAPE(BFN_CASE)
-#undef BFN_CASE
-  return hipErrorInvalidValue;
+                           std::uint32_t min_waves, std::size_t lds_bytes, int* blocks_per_cu) {
+  return with_shape(block, instrumented, min_waves, [&]<int B, int EPT, bool I, int MW>() {
+    return occupancy_relax<B, EPT, I, MW>(persistent, lds_bytes, blocks_per_cu);
+  });
 }
 
 hipError_t prep_launch(const DevGraph& g, const DevBatch& b, std::uint64_t pool_words,

@@ -135,7 +135,10 @@ typedef struct {
                                     compare and an atomic per improved label
                                     (when the LDS holds the proposals);
                                     1 = one global atomic per boundary edge */
-  uint32_t reserved[2];
+  uint32_t min_waves;            /* 0 = compiler default; 6 or 8 = the 256- and
+                                    512-thread kernels compiled for at least that
+                                    many waves per SIMD */
+  uint32_t reserved;
 } bfn_tuning;
 
 /* The reference DeviceGraphLayout32 (device_layout.hpp:72-93) plus the tile

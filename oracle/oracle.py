@@ -324,6 +324,35 @@ def ref_csr_sssp(V, row_offsets, destinations, weights, owner_tiles, tile_select
     return ns, relax.value
 
 
+def ref_csr_sssp_parallel(V, row_offsets, destinations, weights, owner_tiles, tile_count,
+                          query_tiles, sources, threads):
+    """The reference's dijkstra_oracle over many single-source queries, one
+    query per worker thread in waves of `threads` (induced-subgraph
+    construction untimed).  Returns (summed wave wall ns, relaxations)."""
+    lib = ref_lib()
+    if not hasattr(lib, "_csr_parallel_bound"):
+        P = C.c_void_p
+        lib.ref_csr_sssp_parallel.argtypes = [C.c_uint32, P, P, P, P, C.c_uint32, C.c_uint32, P,
+                                              P, P, C.c_uint32, P]
+        lib.ref_csr_sssp_parallel.restype = C.c_int64
+        lib._csr_parallel_bound = True
+    ro = np.ascontiguousarray(row_offsets, np.uint64)
+    de = np.ascontiguousarray(destinations, np.uint32)
+    w = np.ascontiguousarray(weights, np.float32)
+    own = np.ascontiguousarray(owner_tiles, np.uint32)
+    off = np.zeros(len(query_tiles) + 1, np.uint32)
+    off[1:] = np.cumsum([len(t) for t in query_tiles])
+    tl = np.ascontiguousarray(np.concatenate([np.asarray(t, np.uint32) for t in query_tiles]),
+                              np.uint32)
+    src = np.ascontiguousarray(sources, np.uint32)
+    relax = C.c_uint64(0)
+    ns = lib.ref_csr_sssp_parallel(V, _p(ro), _p(de), _p(w), _p(own), tile_count, len(src),
+                                   _p(off), _p(tl), _p(src), threads, C.byref(relax))
+    if ns < 0:
+        raise ValueError(f"reference parallel csr sssp failed: {ns}")
+    return ns, relax.value
+
+
 class RefGraph:
     """Handle over the reference's build_spatially_ordered_graph result."""
 

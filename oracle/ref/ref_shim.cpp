@@ -23,7 +23,9 @@
 #include <cstring>
 #include <exception>
 #include <limits>
+#include <atomic>
 #include <memory>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -90,6 +92,64 @@ void copy_out(T* dst, const S& src) {
 }
 
 }  // namespace
+
+// Induced query graph: the selected tiles' vertices and the edges with both
+// ends selected, built by the reference's build_weighted_graph (provenance =
+// CSR position, so EdgeIds follow CSR order), and the sources renumbered.
+struct InducedQuery {
+  bool valid = false;
+  std::unique_ptr<bfnew::WeightedGraph> graph;
+  std::vector<bfnew::VertexId> sources;
+};
+
+InducedQuery build_induced(std::uint32_t vertex_count, const std::uint64_t* row_offsets,
+                           const std::uint32_t* destinations, const float* weights,
+                           const std::uint32_t* owner_tiles, const std::uint8_t* tile_selected,
+                           const std::uint32_t* sources, std::uint32_t source_count) {
+  constexpr std::uint32_t unmapped = std::numeric_limits<std::uint32_t>::max();
+  std::vector<std::uint32_t> local(vertex_count, unmapped);
+  std::uint32_t kept = 0;
+  for (std::uint32_t v = 0; v < vertex_count; ++v)
+    if (tile_selected[owner_tiles[v]]) local[v] = kept++;
+  std::vector<bfnew::VertexMetadata> vertices(kept,
+                                              bfnew::VertexMetadata::unlocated(bfnew::ResourceClassId{1}));
+  std::vector<bfnew::EdgeInputRecord> edges;
+  for (std::uint32_t s = 0; s < vertex_count; ++s) {
+    if (local[s] == unmapped) continue;
+    for (std::uint64_t p = row_offsets[s]; p < row_offsets[s + 1]; ++p) {
+      const std::uint32_t d = destinations[p];
+      if (local[d] == unmapped) continue;
+      edges.push_back(bfnew::EdgeInputRecord{bfnew::VertexId{local[s]}, bfnew::VertexId{local[d]},
+                                             weights[p], bfnew::PhysicalProvenance{1U, 1U, p}});
+    }
+  }
+  InducedQuery q;
+  q.graph = std::make_unique<bfnew::WeightedGraph>(
+      bfnew::build_weighted_graph(bfnew::InputGraph(std::move(vertices), std::move(edges))));
+  for (std::uint32_t i = 0; i < source_count; ++i) {
+    if (sources[i] >= vertex_count || local[sources[i]] == unmapped) return q;
+    q.sources.push_back(bfnew::VertexId{local[sources[i]]});
+  }
+  q.valid = true;
+  return q;
+}
+
+// Out-edges of reached vertices: what Dijkstra relaxes.
+std::uint64_t reached_out_edges(const bfnew::WeightedGraph& g, const bfnew::SsspResult& result) {
+  std::uint64_t count = 0;
+  const auto out = g.outgoing();
+  for (std::uint32_t s = 0; s < g.vertex_count(); ++s)
+    if (result.distances[s] != std::numeric_limits<float>::infinity())
+      count += out.row_offsets[s + 1] - out.row_offsets[s];
+  return count;
+}
+
+template <typename F>
+void run_threads(std::uint32_t n, F&& f) {
+  std::vector<std::thread> pool;
+  for (std::uint32_t i = 0; i < n; ++i) pool.emplace_back([&f, i] { f(i); });
+  for (auto& t : pool) t.join();
+}
 
 extern "C" {
 
@@ -249,43 +309,60 @@ std::int64_t ref_csr_sssp(std::uint32_t vertex_count, const std::uint64_t* row_o
                           const std::uint32_t* sources, std::uint32_t source_count,
                           std::uint64_t* relaxations) {
   try {
-    constexpr std::uint32_t unmapped = std::numeric_limits<std::uint32_t>::max();
-    std::vector<std::uint32_t> local(vertex_count, unmapped);
-    std::uint32_t kept = 0;
-    for (std::uint32_t v = 0; v < vertex_count; ++v)
-      if (tile_selected[owner_tiles[v]]) local[v] = kept++;
-    std::vector<bfnew::VertexMetadata> vertices(kept,
-                                                bfnew::VertexMetadata::unlocated(bfnew::ResourceClassId{1}));
-    std::vector<bfnew::EdgeInputRecord> edges;
-    for (std::uint32_t s = 0; s < vertex_count; ++s) {
-      if (local[s] == unmapped) continue;
-      for (std::uint64_t p = row_offsets[s]; p < row_offsets[s + 1]; ++p) {
-        const std::uint32_t d = destinations[p];
-        if (local[d] == unmapped) continue;
-        edges.push_back(bfnew::EdgeInputRecord{
-            bfnew::VertexId{local[s]}, bfnew::VertexId{local[d]}, weights[p],
-            bfnew::PhysicalProvenance{1U, 1U, p}});
-      }
-    }
-    bfnew::InputGraph input(std::move(vertices), std::move(edges));
-    const bfnew::WeightedGraph induced = bfnew::build_weighted_graph(input);
-    std::vector<bfnew::VertexId> local_sources;
-    for (std::uint32_t i = 0; i < source_count; ++i) {
-      if (sources[i] >= vertex_count || local[sources[i]] == unmapped) return -1;
-      local_sources.push_back(bfnew::VertexId{local[sources[i]]});
-    }
+    InducedQuery q = build_induced(vertex_count, row_offsets, destinations, weights, owner_tiles,
+                                   tile_selected, sources, source_count);
+    if (!q.valid) return -1;
     const auto start = std::chrono::steady_clock::now();
-    const bfnew::SsspResult result = bfnew::dijkstra_oracle(induced, local_sources);
+    const bfnew::SsspResult result = bfnew::dijkstra_oracle(*q.graph, q.sources);
     const auto stop = std::chrono::steady_clock::now();
-    if (relaxations != nullptr) {
-      std::uint64_t count = 0;
-      const auto out = induced.outgoing();
-      for (std::uint32_t s = 0; s < induced.vertex_count(); ++s)
-        if (result.distances[s] != std::numeric_limits<float>::infinity())
-          count += out.row_offsets[s + 1] - out.row_offsets[s];
-      *relaxations = count;
-    }
+    if (relaxations != nullptr) *relaxations = reached_out_edges(*q.graph, result);
     return std::chrono::duration_cast<std::chrono::nanoseconds>(stop - start).count();
+  } catch (const std::exception&) {
+    return -2;
+  }
+}
+
+// The same Dijkstra over many queries, one query per worker thread, in waves
+// of `threads` queries: each wave builds its induced graphs in parallel
+// (untimed), then runs their Dijkstras in parallel; the returned time is the
+// sum of the waves' Dijkstra wall times.  Query i selects the tiles
+// tiles[tile_offsets[i] .. tile_offsets[i+1]) and has the single source
+// sources[i].  Returns -1 for an invalid query, -2 on error.
+std::int64_t ref_csr_sssp_parallel(std::uint32_t vertex_count, const std::uint64_t* row_offsets,
+                                   const std::uint32_t* destinations, const float* weights,
+                                   const std::uint32_t* owner_tiles, std::uint32_t tile_count,
+                                   std::uint32_t query_count, const std::uint32_t* tile_offsets,
+                                   const std::uint32_t* tiles, const std::uint32_t* sources,
+                                   std::uint32_t threads, std::uint64_t* relaxations) {
+  try {
+    if (threads == 0) threads = 1;
+    std::uint64_t total_ns = 0, total_relax = 0;
+    for (std::uint32_t first = 0; first < query_count; first += threads) {
+      const std::uint32_t n = std::min(threads, query_count - first);
+      std::vector<InducedQuery> wave(n);
+      std::atomic<bool> bad{false};
+      auto build = [&](std::uint32_t i) {
+        std::vector<std::uint8_t> selected(tile_count, 0);
+        const std::uint32_t q = first + i;
+        for (std::uint32_t k = tile_offsets[q]; k < tile_offsets[q + 1]; ++k) selected[tiles[k]] = 1;
+        wave[i] = build_induced(vertex_count, row_offsets, destinations, weights, owner_tiles,
+                                selected.data(), &sources[q], 1);
+        if (!wave[i].valid) bad = true;
+      };
+      run_threads(n, build);
+      if (bad) return -1;
+      std::vector<std::uint64_t> reached(n, 0);
+      const auto start = std::chrono::steady_clock::now();
+      run_threads(n, [&](std::uint32_t i) {
+        const bfnew::SsspResult result = bfnew::dijkstra_oracle(*wave[i].graph, wave[i].sources);
+        reached[i] = reached_out_edges(*wave[i].graph, result);
+      });
+      const auto stop = std::chrono::steady_clock::now();
+      total_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(stop - start).count();
+      for (std::uint64_t r : reached) total_relax += r;
+    }
+    if (relaxations != nullptr) *relaxations = total_relax;
+    return static_cast<std::int64_t>(total_ns);
   } catch (const std::exception&) {
     return -2;
   }

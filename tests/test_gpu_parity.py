@@ -208,7 +208,8 @@ def test_c2_batched_and_plain_bit_exact(device):
     refs = [oracle_query(view, [q.source], batch.tiles[i])[0] for i, q in enumerate(qs)]
     for eng, knobs in ((0, {"sweeps": 0}), (0, {"sweeps": 1}), (1, {"boundary_mode": 0}),
                        (1, {"boundary_mode": 1}), (2, {"boundary_mode": 0}),
-                       (2, {"boundary_mode": 1})):
+                       (2, {"boundary_mode": 1}), (2, {"boundary_mode": 1, "min_waves": 6}),
+                       (1, {"boundary_mode": 1, "min_waves": 8})):
         ws = run(device, dg, batch, eng, bfn.PERSISTENT, **knobs)
         for i in range(len(qs)):
             assert np.array_equal(bits(ws.distances(i)), bits(refs[i])), (eng, knobs, i)
