@@ -3,6 +3,9 @@
 // engines) over the C ABI.
 //
 //   api_test cpu           rules and host layouts only; no device call
+//   api_test planner G.tsv batch planner on the four-query fixture of the
+//                          reference's tests/batch_plan_test.cpp:80-160; G.tsv
+//                          holds that test's literal serialised plan
 //   api_test gpu OUT.bin   runs every engine on the 16x16x64 synthetic graph
 //                          (seed 1), checks the engines against each other,
 //                          and writes the lanes and labels to OUT.bin for the
@@ -11,6 +14,7 @@
 // The reference states the same rules in tests/gpu_api_test.cpp and its
 // batched engine tests; the cases below are this library's own.
 #include "bfnew/batch_layout.hpp"
+#include "bfnew/batch_plan.hpp"
 #include "bfnew/gpu_api.hpp"
 #include "bfnew/hip/engines.hpp"
 #include "bfnew/hip/runtime.hpp"
@@ -22,6 +26,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -328,11 +334,277 @@ int gpu(const char* out_path) {
   return 0;
 }
 
+
+
+// ---------------------------------------------------------------------------
+// Batch planner (reference tests/batch_plan_test.cpp:80-411 states the same
+// fixture and expectations).
+
+struct PlannerFixture {
+  PartitionedGraph partitioned;
+  TileRunLayout64 tile_runs;
+  std::vector<RouteQuery> queries;
+};
+
+PlannerFixture planner_fixture() {
+  const ResourceClassId rc{1U};
+  std::vector<VertexMetadata> vertices{
+      VertexMetadata::located(0, 0, rc),  VertexMetadata::located(1, 0, rc),
+      VertexMetadata::located(10, 0, rc), VertexMetadata::located(30, 0, rc),
+      VertexMetadata::unlocated(rc),      VertexMetadata::located(11, 0, rc),
+      VertexMetadata::located(2, 0, rc)};
+  const std::uint32_t ends[8][2] = {{0, 1}, {0, 2}, {0, 3}, {0, 3}, {0, 4}, {4, 0}, {2, 0}, {1, 2}};
+  std::vector<EdgeInputRecord> edges;
+  for (std::uint32_t i = 0; i < 8; ++i) {
+    EdgeInputRecord e{};
+    e.source = VertexId{ends[i][0]};
+    e.destination = VertexId{ends[i][1]};
+    e.weight = static_cast<float>(i + 1);
+    e.provenance = PhysicalProvenance{i};
+    edges.push_back(e);
+  }
+  const UniformGridPartitioner grid{SpatialOrderConfig{0, 0, 10U, 10U}};
+  PlannerFixture f{grid.partition(InputGraph{std::move(vertices), std::move(edges)}), {}, {}};
+  const WeightedGraph& g = f.partitioned.graph;
+  const auto to_new = g.old_to_new();
+  const auto tile_of = [&](std::uint32_t old) { return g.owner_tiles()[to_new[old].value()]; };
+  const auto query = [&](std::uint32_t id, std::vector<std::uint32_t> src, std::vector<std::uint32_t> tgt,
+                         std::uint32_t gen, std::vector<TileId> tiles) {
+    std::vector<VertexId> s, t;
+    for (auto v : src) s.push_back(to_new[v]);
+    for (auto v : tgt) t.push_back(to_new[v]);
+    RouteQuery q = make_route_query(QueryId{id}, g, s, t, 0U, gen);
+    std::sort(tiles.begin(), tiles.end());
+    q.selected_tiles = tiles;
+    if (!validate_route_query(g, q).ok()) throw std::runtime_error{"invalid planner fixture query"};
+    f.queries.push_back(std::move(q));
+  };
+  query(10, {0, 2}, {5}, 0, {tile_of(0), tile_of(2)});
+  query(20, {0}, {4}, 1, {tile_of(0), tile_of(4)});
+  query(30, {4}, {4}, 2, {tile_of(4)});
+  query(40, {2}, {3}, 3, {tile_of(2), tile_of(3)});
+  f.tile_runs = build_tile_run_layout(g);
+  return f;
+}
+
+BatchPlannerPolicy planner_policy(std::uint32_t width) {
+  BatchPlannerPolicy p;
+  p.lane_width = width;
+  p.maximum_union_inflation_numerator = 3U;
+  p.maximum_union_inflation_denominator = 1U;
+  return p;
+}
+
+ResourceClassId rc_of_test() { return ResourceClassId{1U}; }
+
+void planner(const char* golden_path) {
+  const PlannerFixture fx = planner_fixture();
+  const WeightedGraph& g = fx.partitioned.graph;
+  const SelectedRegionIndex index{g, fx.tile_runs};
+  const auto features = make_batch_query_features(g, index, fx.queries);
+  CHECK(features.size() == 4U);
+  CHECK(features[0].source_count == 2U && features[1].source_count == 1U);
+  CHECK((features[0].source_tiles == std::vector<TileId>{TileId{0U}, TileId{1U}}));
+  CHECK((features[0].target_tiles == std::vector<TileId>{TileId{1U}}));
+  CHECK((features[1].target_tiles == std::vector<TileId>{TileId{3U}}));
+  const std::uint64_t want_v[4] = {5, 4, 1, 3}, want_e[4] = {4, 3, 0, 0};
+  for (int i = 0; i < 4; ++i) {
+    CHECK(features[i].selected_vertex_count == want_v[i]);
+    CHECK(features[i].selected_edge_estimate == want_e[i]);
+    // brute force over the graph: vertices owned by, and edges with both ends
+    // in, the selected tiles
+    const auto& sel = features[i].selected_tiles;
+    const auto in_sel = [&](TileId t) { return std::binary_search(sel.begin(), sel.end(), t); };
+    std::uint64_t nv = 0, ne = 0;
+    const auto out = g.outgoing();
+    for (std::uint32_t v = 0; v < g.vertex_count(); ++v) {
+      if (!in_sel(g.owner_tiles()[v])) continue;
+      ++nv;
+      for (auto e = out.row_offsets[v]; e < out.row_offsets[v + 1]; ++e)
+        ne += in_sel(g.owner_tiles()[out.destinations[e].value()]) ? 1U : 0U;
+    }
+    CHECK(nv == features[i].selected_vertex_count && ne == features[i].selected_edge_estimate);
+  }
+  // An index of another (identical) graph object is refused.
+  const PlannerFixture other = planner_fixture();
+  const SelectedRegionIndex other_index{other.partitioned.graph, other.tile_runs};
+  CHECK(throws<std::invalid_argument>([&] { (void)make_batch_query_features(g, other_index, fx.queries); }));
+
+  // Policy rules and the exact Jaccard.
+  for (std::uint32_t w : {1U, 8U, 16U, 32U})
+    CHECK(validate_batch_planner_policy(planner_policy(w)) == BatchPlannerPolicyError::none);
+  for (std::uint32_t w : {0U, 7U, 64U})
+    CHECK(validate_batch_planner_policy(planner_policy(w)) == BatchPlannerPolicyError::unsupported_lane_width);
+  BatchPlannerPolicy bad;
+  bad.minimum_jaccard_denominator = 0U;
+  CHECK(validate_batch_planner_policy(bad) == BatchPlannerPolicyError::invalid_jaccard_fraction);
+  bad = BatchPlannerPolicy{};
+  bad.maximum_union_inflation_denominator = 0U;
+  CHECK(validate_batch_planner_policy(bad) == BatchPlannerPolicyError::invalid_union_inflation_fraction);
+  const TileId l2[2] = {TileId{0U}, TileId{1U}}, r2[2] = {TileId{1U}, TileId{2U}};
+  const BatchFraction j = tile_set_jaccard(l2, r2);
+  CHECK(j.numerator * 3U == j.denominator);
+
+  // Expanded selection of the same terminals: the larger region anchors.
+  RouteQuery expanded = fx.queries.front();
+  expanded.query_id = QueryId{11U};
+  expanded.expansion_generation = 4U;
+  expanded.selected_tiles.push_back(TileId{2U});
+  const RouteQuery pair[2] = {fx.queries.front(), expanded};
+  const auto pair_features = make_batch_query_features(g, index, pair);
+  const BatchPlan grown = make_overlapping_batch_plan(index, pair_features, planner_policy(8U));
+  CHECK(grown.batches.size() == 1U);
+  CHECK(grown.batches[0].query_ids_by_lane[0] == QueryId{11U});
+  CHECK(grown.batches[0].query_ids_by_lane[1] == QueryId{10U});
+  CHECK((grown.batches[0].union_tile_lane_masks == std::vector<LaneMask>{3U, 3U, 1U}));
+
+  // The golden width-8 plan.
+  const BatchPlan plan = make_overlapping_batch_plan(index, features, planner_policy(8U));
+  CHECK(validate_batch_plan(index, features, plan).ok());
+  CHECK(plan.batches.size() == 1U);
+  const BatchPlanEntry& b = plan.batches[0];
+  CHECK(b.lane_width == 8U && b.valid_lane_mask == 0x0FU);
+  CHECK((b.expansion_generations_by_lane == std::vector<std::uint32_t>{0, 1, 2, 3, 0, 0, 0, 0}));
+  CHECK((b.union_tile_lane_masks == std::vector<LaneMask>{3U, 9U, 8U, 6U}));
+  CHECK(b.union_vertex_count == 7U && b.union_edge_estimate == 8U);
+  CHECK(b.selected_lane_vertex_count == 13U && b.selected_lane_edge_estimate == 7U);
+  const std::uint32_t all_lanes[4] = {0, 1, 2, 3};
+  const BatchFraction infl = batch_union_tile_inflation(features, all_lanes, b.union_tiles);
+  CHECK(infl.numerator * 7U == infl.denominator * 16U);
+  std::ifstream in(golden_path, std::ios::binary);
+  std::stringstream golden;
+  golden << in.rdbuf();
+  CHECK(in.good() || in.eof());
+  const std::string tsv = serialize_batch_plan_tsv(plan);
+  CHECK(!golden.str().empty() && tsv == golden.str());
+  CHECK(fingerprint_batch_plan(plan) == 16700197086403570866ULL);
+  BatchPlan changed = plan;
+  ++changed.batches[0].union_edge_estimate;
+  CHECK(fingerprint_batch_plan(changed) != 16700197086403570866ULL);
+
+  // The plan entry lays out as a lane-batched description in lane order.
+  std::vector<RouteQuery> by_id = fx.queries;
+  std::sort(by_id.begin(), by_id.end(),
+            [](const RouteQuery& x, const RouteQuery& y) { return x.query_id < y.query_id; });
+  BatchDeviceDescription desc;
+  prepare_batch_device_description(g, by_id, features, b, desc);
+  CHECK(desc.lane_width == 8U && desc.valid_lane_mask == 0x0FU);
+  CHECK((desc.query_ids_by_lane == std::vector<std::uint32_t>{10, 20, 30, 40, 0, 0, 0, 0}));
+  CHECK((desc.union_tiles == std::vector<std::uint32_t>{0, 1, 2, 3}));
+  for (std::uint32_t t = 0; t < 4; ++t) CHECK(desc.tile_lane_masks[t] == b.union_tile_lane_masks[t]);
+  CHECK(validate_batch_device_description(g, desc).ok());
+  BatchPlanEntry wrong = b;
+  wrong.query_ids_by_lane[2] = QueryId{40U};
+  CHECK(throws<std::invalid_argument>([&] { prepare_batch_device_description(g, by_id, features, wrong, desc); }));
+
+  // Damaged plans are refused with the matching code.
+  using Code = BatchPlanValidationErrorCode;
+  BatchPlan damaged = plan;
+  damaged.batches[0].union_tile_lane_masks[0] ^= 1U;
+  CHECK(validate_batch_plan(index, features, damaged).code == Code::tile_lane_mask_mismatch);
+  damaged = plan;
+  damaged.batches[0].query_indices_by_lane[5] = 0U;
+  CHECK(validate_batch_plan(index, features, damaged).code == Code::invalid_padding);
+  damaged = plan;
+  damaged.batches[0].query_ids_by_lane[1] = QueryId{99U};
+  CHECK(validate_batch_plan(index, features, damaged).code == Code::query_identity_mismatch);
+  damaged = plan;
+  damaged.batches[0].query_indices_by_lane[1] = 0U;
+  damaged.batches[0].query_ids_by_lane[1] = QueryId{10U};
+  damaged.batches[0].expansion_generations_by_lane[1] = 0U;
+  CHECK(validate_batch_plan(index, features, damaged).code == Code::duplicate_assignment);
+  damaged = plan;
+  ++damaged.batches[0].union_vertex_count;
+  CHECK(validate_batch_plan(index, features, damaged).code == Code::estimate_mismatch);
+  damaged = plan;
+  damaged.batches[0].valid_lane_mask = 0x0BU;
+  CHECK(validate_batch_plan(index, features, damaged).code == Code::invalid_valid_lane_mask);
+
+  // Narrow widths split the corpus; every query lands exactly once and the
+  // family is the same policy at widths 32, 16 and 8.
+  const BatchPlan singles = make_overlapping_batch_plan(index, features, planner_policy(1U));
+  CHECK(singles.batches.size() == 4U && validate_batch_plan(index, features, singles).ok());
+  const BatchPlanFamily family = make_standard_batch_plan_family(index, features, planner_policy(8U));
+  CHECK(family.plans[0].policy.lane_width == 32U && family.plans[2] == plan);
+  for (const BatchPlan& p : family.plans) CHECK(validate_batch_plan(index, features, p).ok());
+
+  // A strict Jaccard floor keeps disjoint regions apart.
+  BatchPlannerPolicy strict = planner_policy(8U);
+  strict.minimum_jaccard_numerator = 1U;
+  strict.minimum_jaccard_denominator = 1U;
+  const BatchPlan apart = make_overlapping_batch_plan(index, features, strict);
+  CHECK(apart.batches.size() == 4U && validate_batch_plan(index, features, apart).ok());
+
+  // A random corpus on a 64x64 grid split into 8x8 tiles: every plan
+  // validates, each admitted lane met the Jaccard floor and the inflation
+  // cap against the union before it, and every query appears exactly once.
+  std::vector<VertexMetadata> rv;
+  std::vector<EdgeInputRecord> re;
+  std::uint64_t state = 0x9E3779B97F4A7C15ULL;
+  const auto next = [&] {
+    state ^= state << 13, state ^= state >> 7, state ^= state << 17;
+    return state;
+  };
+  for (int i = 0; i < 600; ++i)
+    rv.push_back(VertexMetadata::located(static_cast<std::int32_t>(next() % 64),
+                                         static_cast<std::int32_t>(next() % 64), rc_of_test()));
+  for (std::uint32_t i = 0; i < 3000; ++i) {
+    EdgeInputRecord e{};
+    e.source = VertexId{static_cast<std::uint32_t>(next() % 600)};
+    e.destination = VertexId{static_cast<std::uint32_t>(next() % 600)};
+    e.weight = 1.0F + static_cast<float>(next() % 16);
+    e.provenance = PhysicalProvenance{i};
+    re.push_back(e);
+  }
+  const UniformGridPartitioner grid8{SpatialOrderConfig{0, 0, 8U, 8U}};
+  const PartitionedGraph rg = grid8.partition(InputGraph{std::move(rv), std::move(re)});
+  const TileRunLayout64 rruns = build_tile_run_layout(rg.graph);
+  const SelectedRegionIndex rindex{rg.graph, rruns};
+  std::vector<RouteQuery> rq;
+  for (std::uint32_t i = 0; i < 70; ++i) {
+    const VertexId s1[1] = {VertexId{static_cast<std::uint32_t>(next() % 600)}};
+    const VertexId t1[1] = {VertexId{static_cast<std::uint32_t>(next() % 600)}};
+    rq.push_back(make_route_query(QueryId{1000U + 7U * i}, rg.graph, s1, t1,
+                                  static_cast<std::uint32_t>(next() % 3), static_cast<std::uint32_t>(i % 5)));
+  }
+  const auto rf = make_batch_query_features(rg.graph, rindex, rq);
+  for (std::uint32_t w : {8U, 16U, 32U}) {
+    const BatchPlannerPolicy pol = planner_policy(w);
+    const BatchPlan rp = make_overlapping_batch_plan(rindex, rf, pol);
+    CHECK(validate_batch_plan(rindex, rf, rp).ok());
+    std::vector<int> seen(rf.size(), 0);
+    for (const BatchPlanEntry& e : rp.batches) {
+      std::vector<TileId> uni;
+      std::uint64_t lane_tiles = 0;
+      for (std::uint32_t lane = 0; lane < w; ++lane) {
+        if (!(e.valid_lane_mask >> lane & 1U)) break;
+        const auto& f = rf[e.query_indices_by_lane[lane]];
+        ++seen[e.query_indices_by_lane[lane]];
+        std::vector<TileId> next_uni;
+        std::set_union(uni.begin(), uni.end(), f.selected_tiles.begin(), f.selected_tiles.end(),
+                       std::back_inserter(next_uni));
+        if (lane > 0) {
+          std::vector<TileId> both;
+          std::set_intersection(uni.begin(), uni.end(), f.selected_tiles.begin(),
+                                f.selected_tiles.end(), std::back_inserter(both));
+          // |both| / |next_uni| >= 1/8 and (lane+1)|next_uni| / tiles <= 3
+          CHECK(8U * both.size() >= next_uni.size());
+          CHECK((lane + 1U) * next_uni.size() <= 3U * (lane_tiles + f.selected_tiles.size()));
+        }
+        lane_tiles += f.selected_tiles.size();
+        uni = std::move(next_uni);
+      }
+      CHECK(uni == e.union_tiles);
+    }
+    CHECK(std::all_of(seen.begin(), seen.end(), [](int n) { return n == 1; }));
+  }
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
   if (argc < 2) {
-    std::fprintf(stderr, "usage: api_test cpu | gpu OUT.bin\n");
+    std::fprintf(stderr, "usage: api_test cpu | planner GOLDEN.tsv | gpu OUT.bin\n");
     return 2;
   }
   int rc = 0;
@@ -340,6 +612,8 @@ int main(int argc, char** argv) {
     if (std::strcmp(argv[1], "cpu") == 0) {
       cpu_rules();
       cpu_layouts();
+    } else if (std::strcmp(argv[1], "planner") == 0 && argc >= 3) {
+      planner(argv[2]);
     } else if (std::strcmp(argv[1], "gpu") == 0 && argc >= 3) {
       rc = gpu(argv[2]);
     } else {

@@ -1,7 +1,7 @@
 """The reference-named C++ API (tests/cpp/api_test.cpp driver).
 
-CPU: run options / controller / run status rules, resident-graph plans and
-batch descriptions.  GPU: every batched engine, both control modes, compact
+CPU: run options / controller / run status rules, resident-graph plans,
+batch descriptions and the batch planner.  GPU: every batched engine, both control modes, compact
 status, compact paths and the three single-query engines on the 16x16x64
 graph (seed 1); the driver checks the engines against each other and this
 test checks the labels and paths it writes against the oracle.
@@ -30,6 +30,22 @@ def _run(*args, timeout=120):
 def test_cpp_api_rules_and_layouts():
     out = _run("cpu")
     assert "0 failure(s)" in out.stdout
+
+
+def test_cpp_batch_planner_golden():
+    """Greedy overlapping planner on the reference's four-query fixture
+    (tests/batch_plan_test.cpp:80-411): lane order, union masks, estimates,
+    the literal serialised plan (tests/golden/batch_plan_golden.tsv, whose
+    FNV-1a fingerprint is the reference's 16700197086403570866), damaged-plan
+    rejection codes and a random 70-query corpus at widths 8/16/32."""
+    golden = os.path.join(ROOT, "tests", "golden", "batch_plan_golden.tsv")
+    data = open(golden, "rb").read()
+    h = 14695981039346656037
+    for c in data:
+        h = ((h ^ c) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    assert h == 16700197086403570866
+    out = _run("planner", golden)
+    assert "planner: 0 failure(s)" in out.stdout
 
 
 def _read(path):
