@@ -67,8 +67,8 @@ def profiled_traffic(workload, tune):
     if not entry:
         return None, None
     cfg = entry["config"]
-    keys = ("engine", "block", "sweeps", "boundary_mode", "use_graph")
-    if any(int(cfg.get(k, 0)) != int(tune.get(k, 0)) for k in keys):
+    keys = ("engine", "block", "sweeps", "boundary_mode", "use_graph", "min_waves", "window")
+    if any(float(cfg.get(k, 0)) != float(tune.get(k, 0)) for k in keys):
         return None, None
     return entry["hbm_bytes_per_step"], entry["source"]
 
@@ -89,7 +89,7 @@ def make_batch(hg, cfg):
 def apply_tuning(batch, qs, tune_entry):
     t = dict(engine=bfn.JACOBI_PULL, block=256, sweeps=0, control=bfn.PERSISTENT,
              classes=None, items_per_claim=1, use_graph=0, graph_rounds=0, boundary_mode=0,
-             min_waves=0)
+             min_waves=0, window=0.0)
     if tune_entry:
         t.update(tune_entry)
     if t.get("classes"):
@@ -107,7 +107,7 @@ def timed_steps(ws, steps, warmup):
         ws.rerun()
         relax_ms.append(ws.timing()[0])
     bfn.hip_lib().bfn_device_synchronize(ws.device.h)
-    return time.perf_counter() - t0, relax_ms
+    return time.perf_counter() - t0, relax_ms, ws.timing()[2]
 
 
 def cpu_baseline(hg_view, qs, tiles, budget_s):
@@ -175,7 +175,7 @@ def plain_vs_batched(dev, steps):
               bfn.tuning(tile_sweeps=0, count_relaxations=1))
     _, per, _ = ws.status()
     relax = [p.relaxations for p in per]
-    elapsed, _ = timed_steps(ws, steps, 1)
+    elapsed, _, _ = timed_steps(ws, steps, 1)
     batched = sum(relax) * steps / elapsed
     out = {}
     for eng in (bfn.JACOBI_PULL, bfn.DENSE_CHAOTIC_PUSH, bfn.FRONTIER_PUSH):
@@ -286,7 +286,7 @@ def negotiated(dev, tune):
         ws.submit(dg, batch, opts, bfn.tuning(tile_sweeps=tune["sweeps"], count_relaxations=1,
                                               items_per_claim=tune["items_per_claim"],
                                               boundary_mode=tune["boundary_mode"],
-                                              min_waves=tune["min_waves"]))
+                                              min_waves=tune["min_waves"], window=tune["window"]))
         st, per, _ = ws.status()
         relax_ms = ws.timing()[0]
         relaxations = sum(p.relaxations for p in per)
@@ -367,7 +367,7 @@ def main():
               bfn.tuning(tile_sweeps=tune["sweeps"], count_relaxations=1,
                          items_per_claim=tune["items_per_claim"], use_graph=tune["use_graph"],
                          graph_rounds=tune["graph_rounds"],
-                         boundary_mode=tune["boundary_mode"], min_waves=tune["min_waves"]))
+                         boundary_mode=tune["boundary_mode"], min_waves=tune["min_waves"], window=tune["window"]))
     st, per, _ = ws.status()
     if not st.converged:
         raise SystemExit("batch did not converge")
@@ -379,8 +379,10 @@ def main():
                                           use_graph=tune["use_graph"],
                                           graph_rounds=tune["graph_rounds"],
                                           boundary_mode=tune["boundary_mode"],
-                                          min_waves=tune["min_waves"]))
-    elapsed, relax_ms = timed_steps(ws, args.steps, args.warmup)
+                                          min_waves=tune["min_waves"], window=tune["window"]))
+    elapsed, relax_ms, launches = timed_steps(ws, args.steps, args.warmup)
+    # one round kernel + one advance kernel per host-polled or replayed round
+    round_launches = 1 if tune["control"] == bfn.PERSISTENT and not tune["use_graph"] else launches // 2
     value = relax_per_step * args.steps / elapsed
     ms_per_step = elapsed * 1e3 / args.steps
     kernel_ms = float(np.mean(relax_ms))
@@ -417,16 +419,21 @@ def main():
                    "unbounded_fraction": C3["unbounded"], "engine": bfn.ENGINE_NAMES[tune["engine"]],
                    "per_class_engines": tune.get("classes"), "block": tune["block"],
                    "tile_sweeps": tune["sweeps"], "boundary_mode": tune["boundary_mode"],
-                   "min_waves": tune["min_waves"],
+                   "min_waves": tune["min_waves"], "window": tune["window"],
                    "control": bfn.CONTROL_NAMES[tune["control"]] +
                    (f"+hipgraph({tune['graph_rounds']} rounds)" if tune["use_graph"] else ""),
                    "relaxations_per_step": relax_per_step},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_source,
-                     "kernel": "relax (tile visits)", "kernel_ms": kernel_ms,
-                     "algorithmic_bytes_per_launch": counters.algorithmic_bytes,
-                     "edges_relaxed_per_launch": counters.edges_relaxed},
+                     "kernel": "relax_round_kernel" if round_launches > 1 else
+                     "relax_persistent_kernel",
+                     "per": "batch step: every round launch of one relaxation (achieved and "
+                            "traffic are bytes per step over kernel time per step)",
+                     "kernel_ms_per_step": kernel_ms, "round_launches_per_step": round_launches,
+                     "avg_launch_us": kernel_ms * 1e3 / round_launches,
+                     "algorithmic_bytes_per_step": counters.algorithmic_bytes,
+                     "edges_relaxed_per_step": counters.edges_relaxed},
         "visit_profile": visit_profile,
     }
     if not args.no_plain:

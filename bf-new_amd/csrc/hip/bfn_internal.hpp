@@ -132,6 +132,7 @@ struct DevBatch {
   std::uint32_t instrumentation;
   std::uint32_t boundary_mode;    // push: 0 LDS proposals when they fit, 1 direct atomics
   std::uint32_t min_waves;        // kernel variant bounded to >= this many waves per SIMD
+  float window;                   // push: label window of eligible slots, 0 = all queued
   const std::uint32_t* tile_begin;      // [Q]   first position of q in query_tiles
   const std::uint32_t* tile_end;        // [Q]   one past q's last position
   const std::uint32_t* query_tiles;     // [.]   q's selected tiles, sorted
@@ -153,6 +154,11 @@ struct DevBatch {
   std::uint32_t* dist;                  // float bits pool
   std::uint32_t* last_change;           // [Q] 1 + last round with a decrease
   std::uint32_t* pending;               // [Q] queued work items of q, or null
+  // Label window (push engines, window > 0): smallest label pushed into each
+  // slot for round r, slot_min[(r & 1) * S + s], and the smallest of those
+  // over q's queued slots, query_min[(r % 3) * Q + q].  Null when unused.
+  std::uint32_t* slot_min;              // [2*S]
+  std::uint32_t* query_min;             // [3*Q]
   DevControl* control;
 };
 
@@ -227,6 +233,19 @@ __device__ __forceinline__ void enqueue(const DevBatch& b, std::uint32_t slot,
     b.lists[static_cast<std::size_t>(li) * b.slot_count + pos] = slot;
     if (b.pending) atomic_add_agent(&b.pending[b.slot_query[slot]], 1u);
   }
+}
+
+// Queues slot for `round` with the smallest label pushed into it (the label
+// window of the push engines orders slots by it).
+__device__ __forceinline__ void enqueue_pushed(const DevBatch& b, std::uint32_t slot,
+                                               std::uint32_t round, std::uint32_t label) {
+  if (b.slot_min) {
+    atomic_min_agent(&b.slot_min[static_cast<std::size_t>(round & 1) * b.slot_count + slot], label);
+    atomic_min_agent(&b.query_min[static_cast<std::size_t>(round % 3) * b.query_count +
+                                  b.slot_query[slot]],
+                     label);
+  }
+  enqueue(b, slot, round);
 }
 
 // Position of key in the sorted range values[begin, end), or kNone.

@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -162,7 +163,7 @@ struct bfn_workspace_s {
   // batch metadata in one device buffer + separately grown state
   DeviceBuffer meta, slot_in, slot_out, stamp, lists, pool, last_change, control, status, relax,
       scratch_v, paths_summary, paths_len, paths_off, frames, onpath, clean, arena_v, arena_l,
-      arena_e, occupancy;
+      arena_e, occupancy, window_min;
   PinnedBuffer staging, h_status, h_control, h_paths;
   // last batch
   bfn_graph graph = nullptr;
@@ -705,7 +706,7 @@ std::size_t meta_layout(std::uint32_t Q, std::uint32_t S, std::uint32_t ns, std:
 std::uint64_t batch_bytes(const BatchPlanHost& p, std::uint64_t meta_bytes) {
   return meta_bytes + 4ull * (p.in_total + p.out_total) + 16ull * p.S +
          4ull * p.pool_words + 4ull * p.Q * 2 + sizeof(QueryStatusDev) * p.Q + 8ull * p.Q +
-         sizeof(DevControl);
+         sizeof(DevControl) + 8ull * p.S + 12ull * p.Q;  // last two: the label window
 }
 
 // Sets up the DevBatch view over the workspace buffers.
@@ -721,6 +722,7 @@ void bind_view(bfn_workspace ws, const BatchPlanHost& p, const std::size_t* seg,
   b.instrumentation = o->instrumentation;
   b.boundary_mode = t->boundary_mode;
   b.min_waves = t->min_waves;
+  b.window = p.any_push ? t->window : 0.0f;
   b.tile_begin = reinterpret_cast<const std::uint32_t*>(m + seg[0]);
   b.tile_end = b.tile_begin + 1;
   b.query_tiles = reinterpret_cast<const std::uint32_t*>(m + seg[1]);
@@ -742,6 +744,8 @@ void bind_view(bfn_workspace ws, const BatchPlanHost& p, const std::size_t* seg,
   b.dist = ws->pool.as<std::uint32_t>();
   b.last_change = ws->last_change.as<std::uint32_t>();
   b.pending = nullptr;
+  b.slot_min = b.window > 0.0f ? ws->window_min.as<std::uint32_t>() : nullptr;
+  b.query_min = b.slot_min ? b.slot_min + 2ull * p.S : nullptr;
   b.control = ws->control.as<DevControl>();
 }
 
@@ -888,9 +892,10 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
   if (int32_t rc = validate_options(o)) return rc;
   bfn_tuning t;
   if (tuning) t = *tuning; else bfn_tuning_default(&t);
-  if (t.tile_sweeps > 1 || t.boundary_mode > 1 || t.use_graph > 1 || t.min_waves > 8)
+  if (t.tile_sweeps > 1 || t.boundary_mode > 1 || t.use_graph > 1 || t.min_waves > 8 ||
+      !(t.window >= 0.0f) || !std::isfinite(t.window))
     return fail(BFN_E_INVALID_ARGUMENT,
-                "invalid tuning (tile_sweeps, boundary_mode, use_graph, min_waves)");
+                "invalid tuning (tile_sweeps, boundary_mode, use_graph, min_waves, window)");
   BatchPlanHost p;
   if (int32_t rc = plan_batch(graph, d, o, p)) return rc;
   BFN_HIP(hipSetDevice(ws->device->ordinal));
@@ -930,6 +935,7 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
   BFN_HIP(ws->lists.reserve(12ull * std::max(S, 1u)));
   BFN_HIP(ws->pool.reserve(4ull * std::max<std::uint64_t>(p.pool_words, 1)));
   BFN_HIP(ws->last_change.reserve(4ull * Q));
+  if (p.any_push && t.window > 0.0f) BFN_HIP(ws->window_min.reserve(8ull * S + 12ull * Q));
   BFN_HIP(ws->control.reserve(sizeof(DevControl)));
   BFN_HIP(ws->status.reserve(sizeof(QueryStatusDev) * Q));
   BFN_HIP(ws->relax.reserve(8ull * Q));

@@ -265,7 +265,7 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
   for (std::uint32_t k = tid; k < KO; k += B) {
     const std::uint32_t ns = b.slot_out[out_base + k];
     l.table[k] = ns;
-    l.table[256 + k] = 0;
+    l.table[256 + k] = kInfBits;  // smallest label pushed into neighbour k
     if (proposals) {
       l.table[512 + k] = g.out_nbr_stage[ob0 + k];
     } else if (ns != kNone) {
@@ -365,16 +365,16 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
       const std::uint32_t* slab = l.prop + l.table[512 + k];
       std::uint32_t* target = b.dist + b.slot_dist[ns];
       if (tid == 0) st.bytes += 4ull * n;  // the neighbour slab, read once
-      bool hit = false;
+      std::uint32_t best = kInfBits;
       for (std::uint32_t i = tid; i < n; i += B) {
         const std::uint32_t pv = slab[i];
         if (pv != kInfBits && pv < target[i]) {
           atomic_min_agent(target + i, pv);
           ++suc;
-          hit = true;
+          best = min(best, pv);
         }
       }
-      if (hit) l.table[256 + k] = 1;
+      if (best != kInfBits) atomicMin(&l.table[256 + k], best);
     }
   } else {
     // One global atomic per boundary candidate that beats the neighbour's
@@ -408,7 +408,7 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
         if (cb < seen[j]) {
           atomic_min_agent(target, cb);
           ++suc;
-          l.table[256 + pk_k(chunk.pack[j])] = 1;
+          atomicMin(&l.table[256 + pk_k(chunk.pack[j])], cb);
         }
       }
     }
@@ -431,8 +431,8 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
   const bool pushed_any = __syncthreads_or(changed);
   bool marked = false;
   for (std::uint32_t k = 1 + tid; k < KO; k += B) {
-    if (l.table[256 + k]) {
-      enqueue(b, l.table[k], r + 1);
+    if (l.table[256 + k] != kInfBits) {
+      enqueue_pushed(b, l.table[k], r + 1, l.table[256 + k]);
       marked = true;
     }
   }
@@ -445,11 +445,39 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
 // ---------------------------------------------------------------------------
 // Round processing shared by the persistent and host-polled kernels.
 // ---------------------------------------------------------------------------
+
+// Label window of the push engines: a queued slot is visited in round r only
+// when the smallest label pushed into it is below the query's smallest such
+// label over its queued slots plus the window; otherwise it moves, with its
+// label, to round r + 1.  The slot holding the query's minimum is always
+// visited, so every query advances each round, and any visiting order
+// reaches the same fixed point.  Block-uniform result.
+__device__ __forceinline__ bool in_window(const DevBatch& b, std::uint32_t s, std::uint32_t r,
+                                          const Lds& l) {
+  if (threadIdx.x == 0) {
+    const std::uint32_t q = b.slot_query[s];
+    std::uint32_t* own = b.slot_min + static_cast<std::size_t>(r & 1) * b.slot_count + s;
+    const std::uint32_t m = load_agent(own);
+    const float lo = as_float(load_agent(&b.query_min[static_cast<std::size_t>(r % 3) * b.query_count + q]));
+    const bool go = m == kInfBits || as_float(m) < lo + b.window;
+    store_agent(own, kInfBits);  // consumed: the round r + 2 pushes start over
+    if (!go) enqueue_pushed(b, s, r + 1, m);
+    l.flags[1] = go ? 1u : 0u;
+  }
+  __syncthreads();
+  return l.flags[1] != 0;
+}
+
 template <int B, int EPT, bool I>
 __device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch& b, std::uint32_t r,
                               std::uint32_t n, const Lds& l) {
   const std::uint32_t li = r % 3;
   const std::uint32_t chunk = b.items_per_claim ? b.items_per_claim : 1u;
+  if (b.slot_min && blockIdx.x == 0) {
+    // query minima of round r + 2: neither read nor written in round r
+    std::uint32_t* reuse = b.query_min + static_cast<std::size_t>((r + 2) % 3) * b.query_count;
+    for (std::uint32_t q = threadIdx.x; q < b.query_count; q += B) reuse[q] = kInfBits;
+  }
   RecordRegs<B, EPT> regs;
   std::uint32_t regs_key = kNone;  // tile and orientation of the records in regs
   VisitStats<I> st;
@@ -465,10 +493,11 @@ __device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch&
       const std::uint32_t engine = b.engine[b.slot_query[s]];
       if (engine == kJacobiPull) {
         visit_pull<B, EPT, I>(g, b, s, r, l, regs, regs_key, st);
-      } else {
+        ++st.items;
+      } else if (!b.slot_min || in_window(b, s, r, l)) {
         visit_push<B, EPT, I>(g, b, s, r, engine == kFrontierPush, l, regs, regs_key, st);
+        ++st.items;
       }
-      ++st.items;
       __syncthreads();
       // the visit's own enqueues are issued before its item is retired
       if (b.pending && threadIdx.x == 0) atomic_add_agent(&b.pending[b.slot_query[s]], 0xFFFFFFFFu);
@@ -646,7 +675,7 @@ __global__ void seed_kernel(DevGraph g, DevBatch b) {
     std::uint32_t* column = b.dist + b.slot_dist[s];
     column[u - v0] = 0u;
     if (jacobi) column[npt + (u - v0)] = 0u;
-    enqueue(b, s, 0);
+    if (jacobi) enqueue(b, s, 0); else enqueue_pushed(b, s, 0, 0u);
     if (jacobi) {
       const std::uint32_t ob = b.slot_out_base[s];
       const std::uint32_t KO = g.out_nbr_offsets[T + 1] - g.out_nbr_offsets[T];
@@ -744,6 +773,9 @@ hipError_t prep_launch(const DevGraph& g, const DevBatch& b, std::uint64_t pool_
   if (b.slot_count)
     hipLaunchKernelGGL(prep_slots_kernel, dim3((b.slot_count + 255) / 256), dim3(256), 0,
                        stream, g, b);
+  if (b.slot_min)
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(256), dim3(256), 0, stream, b.slot_min,
+                       2ull * b.slot_count + 3ull * b.query_count, kInfBits);
   if (b.query_count)
     hipLaunchKernelGGL(seed_kernel, dim3((b.query_count + 127) / 128), dim3(128), 0, stream, g,
                        b);

@@ -138,7 +138,11 @@ typedef struct {
   uint32_t min_waves;            /* 0 = compiler default; 6 or 8 = the 256- and
                                     512-thread kernels compiled for at least that
                                     many waves per SIMD */
-  uint32_t reserved;
+  float window;                  /* push engines: visit a queued tile only when
+                                    the smallest label pushed into it is below
+                                    its query's smallest queued label plus this
+                                    window (0 = every queued tile each round);
+                                    any window gives the same distances */
 } bfn_tuning;
 
 /* The reference DeviceGraphLayout32 (device_layout.hpp:72-93) plus the tile
