@@ -93,11 +93,11 @@ def candidates(engines, blocks, quick):
     out = []
     for engine, block in itertools.product(engines, blocks):
         jacobi = engine == bfn.JACOBI_PULL
-        waves = (0, 6) if (not jacobi and block in (256, 512)) else (0,)
+        waves = (0, 6, 8) if (not jacobi and block in (256, 512)) else (0,)
         for sw in ((0, 1) if jacobi else (0,)):
             for mode in ((0,) if jacobi else (0, 1)):
                 for mw in waves:
-                    for ipc in ((2,) if quick else (1, 2, 4)):
+                    for ipc in ((2,) if quick else (1, 2)):
                         out.append(dict(engine=engine, block=block, sweeps=sw, items_per_claim=ipc,
                                         boundary_mode=mode, min_waves=mw,
                                         control=bfn.PERSISTENT))

@@ -133,6 +133,7 @@ struct DevBatch {
   std::uint32_t boundary_mode;    // push: 0 LDS proposals when they fit, 1 direct atomics
   std::uint32_t min_waves;        // kernel variant bounded to >= this many waves per SIMD
   float window;                   // push: label window of eligible slots, 0 = all queued
+  std::uint32_t engine_kinds;     // engines present: 1 Jacobi, 2 push, 3 both
   const std::uint32_t* tile_begin;      // [Q]   first position of q in query_tiles
   const std::uint32_t* tile_end;        // [Q]   one past q's last position
   const std::uint32_t* query_tiles;     // [.]   q's selected tiles, sorted

@@ -723,6 +723,7 @@ void bind_view(bfn_workspace ws, const BatchPlanHost& p, const std::size_t* seg,
   b.boundary_mode = t->boundary_mode;
   b.min_waves = t->min_waves;
   b.window = p.any_push ? t->window : 0.0f;
+  b.engine_kinds = (p.any_jacobi ? 1u : 0u) | (p.any_push ? 2u : 0u);
   b.tile_begin = reinterpret_cast<const std::uint32_t*>(m + seg[0]);
   b.tile_end = b.tile_begin + 1;
   b.query_tiles = reinterpret_cast<const std::uint32_t*>(m + seg[1]);
@@ -977,6 +978,7 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
                                              std::to_string(lds_limit) + " bytes)");
   int per_cu = 0;
   BFN_HIP(relax_occupancy(ws->block, o->control_mode == 0, o->instrumentation != 0, t.min_waves,
+                          ws->view.engine_kinds,
                           ws->lds_bytes, &per_cu));
   if (per_cu <= 0) return fail(BFN_E_NOT_REPRESENTABLE, "relaxation kernel cannot be resident");
   const std::uint32_t cus = static_cast<std::uint32_t>(ws->device->props.multiProcessorCount);

@@ -496,6 +496,7 @@ int32_t bfn_server_create(bfn_device device, bfn_graph graph, const bfn_server_o
   b.instrumentation = 0;
   b.boundary_mode = 1;
   b.min_waves = 0;
+  b.engine_kinds = o->engine == kJacobiPull ? 1u : 2u;  // one engine per server
   // A request's slots form one contiguous range of the arena, in tile
   // order, so positions and slots coincide.
   b.tile_begin = sv->d_begin;
@@ -525,7 +526,8 @@ int32_t bfn_server_create(bfn_device device, bfn_graph graph, const bfn_server_o
                      sv->Scap, sv->slab_words, sv->kin, sv->kout, sv->d_active, sv->d_src_off,
                      sv->d_tgt_off, sv->d_clean);
   int per_cu = 0;
-  if (relax_occupancy(sv->block, false, false, 0, sv->lds_bytes, &per_cu) != hipSuccess ||
+  if (relax_occupancy(sv->block, false, false, 0, o->engine == kJacobiPull ? 1u : 2u,
+                      sv->lds_bytes, &per_cu) != hipSuccess ||
       per_cu <= 0)
     return bail(BFN_E_NOT_REPRESENTABLE, "relaxation kernel cannot be resident");
   sv->grid = static_cast<std::uint32_t>(per_cu) * device_cu_count(device);
