@@ -123,6 +123,17 @@ struct DevControl {
   unsigned long long phase_cycles[kPhaseCount];  // summed over workgroups
 };
 
+// What a visit needs before its first label load, per slot, in one record:
+// the push engines' out-neighbour orientation, Jacobi's in-neighbour one.
+struct SlotDesc {
+  std::uint32_t tile, query, engine, v0;
+  std::uint32_t npt, nbr0, nbr_count, nbr_base;  // neighbour-tile list of the tile, slot's list
+  std::uint32_t e0, ei, e1, pad;                 // records: internal [e0, ei), boundary [ei, e1)
+  unsigned long long dist;                       // pool offset of the slot's labels
+  unsigned long long pad2;
+};
+static_assert(sizeof(SlotDesc) == 64, "one 64-byte record per slot");
+
 struct DevBatch {
   std::uint32_t query_count;
   std::uint32_t slot_count;
@@ -150,6 +161,8 @@ struct DevBatch {
   const std::uint32_t* slot_out_base;   // [S]
   std::uint32_t* slot_in;
   std::uint32_t* slot_out;
+  SlotDesc* slot_desc;                  // [S]
+  unsigned long long* slot_out_off;     // pool offset of each slot_out entry, ~0 for none
   std::uint32_t* stamp;                 // [S]
   std::uint32_t* lists;                 // [3*S]
   std::uint32_t* dist;                  // float bits pool
@@ -247,6 +260,37 @@ __device__ __forceinline__ void enqueue_pushed(const DevBatch& b, std::uint32_t 
                      label);
   }
   enqueue(b, slot, round);
+}
+
+// Writes slot s's descriptor and the pool offsets of its out-neighbour
+// slots (slot_out must be resolved).
+__device__ __forceinline__ void fill_slot_desc(const DevGraph& g, const DevBatch& b,
+                                               std::uint32_t s) {
+  SlotDesc d;
+  d.tile = b.tiles[s];
+  d.query = b.slot_query[s];
+  d.engine = b.engine[d.query];
+  d.v0 = g.tile_vertex_offsets[d.tile];
+  d.npt = g.tile_vertex_offsets[d.tile + 1] - d.v0;
+  const bool pull = d.engine == kJacobiPull;
+  const std::uint32_t* nbr = pull ? g.in_nbr_offsets : g.out_nbr_offsets;
+  d.nbr0 = nbr[d.tile];
+  d.nbr_count = nbr[d.tile + 1] - d.nbr0;
+  d.nbr_base = pull ? b.slot_in_base[s] : b.slot_out_base[s];
+  const std::uint32_t* offsets = pull ? g.column_offsets : g.row_offsets;
+  d.e0 = offsets[d.v0];
+  d.ei = d.e0 + (pull ? g.csc_intra : g.csr_intra)[d.tile];
+  d.e1 = offsets[d.v0 + d.npt];
+  d.pad = 0;
+  d.dist = b.slot_dist[s];
+  d.pad2 = 0;
+  b.slot_desc[s] = d;
+  const std::uint32_t ob = b.slot_out_base[s];
+  const std::uint32_t ko = g.out_nbr_offsets[d.tile + 1] - g.out_nbr_offsets[d.tile];
+  for (std::uint32_t k = 0; k < ko; ++k) {
+    const std::uint32_t ns = b.slot_out[ob + k];
+    b.slot_out_off[ob + k] = ns == kNone ? ~0ull : b.slot_dist[ns];
+  }
 }
 
 // Position of key in the sorted range values[begin, end), or kNone.

@@ -163,7 +163,7 @@ struct bfn_workspace_s {
   // batch metadata in one device buffer + separately grown state
   DeviceBuffer meta, slot_in, slot_out, stamp, lists, pool, last_change, control, status, relax,
       scratch_v, paths_summary, paths_len, paths_off, frames, onpath, clean, arena_v, arena_l,
-      arena_e, occupancy, window_min;
+      arena_e, occupancy, window_min, slot_desc, slot_out_off;
   PinnedBuffer staging, h_status, h_control, h_paths;
   // last batch
   bfn_graph graph = nullptr;
@@ -706,7 +706,8 @@ std::size_t meta_layout(std::uint32_t Q, std::uint32_t S, std::uint32_t ns, std:
 std::uint64_t batch_bytes(const BatchPlanHost& p, std::uint64_t meta_bytes) {
   return meta_bytes + 4ull * (p.in_total + p.out_total) + 16ull * p.S +
          4ull * p.pool_words + 4ull * p.Q * 2 + sizeof(QueryStatusDev) * p.Q + 8ull * p.Q +
-         sizeof(DevControl) + 8ull * p.S + 12ull * p.Q;  // last two: the label window
+         sizeof(DevControl) + 8ull * p.S + 12ull * p.Q +  // the label window
+         sizeof(SlotDesc) * p.S + 8ull * p.out_total;    // slot descriptors
 }
 
 // Sets up the DevBatch view over the workspace buffers.
@@ -740,6 +741,8 @@ void bind_view(bfn_workspace ws, const BatchPlanHost& p, const std::size_t* seg,
   b.slot_out_base = reinterpret_cast<const std::uint32_t*>(m + seg[10]);
   b.slot_in = ws->slot_in.as<std::uint32_t>();
   b.slot_out = ws->slot_out.as<std::uint32_t>();
+  b.slot_desc = ws->slot_desc.as<SlotDesc>();
+  b.slot_out_off = ws->slot_out_off.as<unsigned long long>();
   b.stamp = ws->stamp.as<std::uint32_t>();
   b.lists = ws->lists.as<std::uint32_t>();
   b.dist = ws->pool.as<std::uint32_t>();
@@ -932,6 +935,8 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
   BFN_HIP(ws->meta.reserve(meta_bytes));
   BFN_HIP(ws->slot_in.reserve(4ull * std::max<std::uint64_t>(p.in_total, 1)));
   BFN_HIP(ws->slot_out.reserve(4ull * std::max<std::uint64_t>(p.out_total, 1)));
+  BFN_HIP(ws->slot_desc.reserve(sizeof(SlotDesc) * std::max(S, 1u)));
+  BFN_HIP(ws->slot_out_off.reserve(8ull * std::max<std::uint64_t>(p.out_total, 1)));
   BFN_HIP(ws->stamp.reserve(4ull * std::max(S, 1u)));
   BFN_HIP(ws->lists.reserve(12ull * std::max(S, 1u)));
   BFN_HIP(ws->pool.reserve(4ull * std::max<std::uint64_t>(p.pool_words, 1)));

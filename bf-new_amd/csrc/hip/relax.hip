@@ -53,6 +53,7 @@ __global__ void prep_slots_kernel(DevGraph g, DevBatch b) {
   const std::uint32_t o0 = g.out_nbr_offsets[T], o1 = g.out_nbr_offsets[T + 1];
   const std::uint32_t obs = b.slot_out_base[s];
   for (std::uint32_t k = o0; k < o1; ++k) b.slot_out[obs + (k - o0)] = find_slot(b, q, g.out_nbr_tiles[k]);
+  fill_slot_desc(g, b, s);
   b.stamp[s] = 0;
   b.last_change[q] = 0;
 }

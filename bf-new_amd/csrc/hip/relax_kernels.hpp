@@ -127,18 +127,18 @@ __device__ __forceinline__ std::uint32_t pk_dst(std::uint32_t p) { return p & kP
 // ---------------------------------------------------------------------------
 template <int B, int EPT, bool I>
 __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b, std::uint32_t s,
-                           std::uint32_t r, const Lds& l, RecordRegs<B, EPT>& internal,
-                           std::uint32_t& regs_key, VisitStats<I>& st) {
+                           const SlotDesc& d, std::uint32_t r, const Lds& l,
+                           RecordRegs<B, EPT>& internal, std::uint32_t& regs_key,
+                           VisitStats<I>& st) {
   const std::uint32_t tid = threadIdx.x;
   st.start();
-  const std::uint32_t T = b.tiles[s];
-  const std::uint32_t q = b.slot_query[s];
-  const std::uint32_t v0 = g.tile_vertex_offsets[T];
-  const std::uint32_t npt = g.tile_vertex_offsets[T + 1] - v0;
-  const std::uint32_t nb0 = g.in_nbr_offsets[T];
-  const std::uint32_t K = g.in_nbr_offsets[T + 1] - nb0;
+  const std::uint32_t T = d.tile;
+  const std::uint32_t q = d.query;
+  const std::uint32_t npt = d.npt;
+  const std::uint32_t nb0 = d.nbr0;
+  const std::uint32_t K = d.nbr_count;
   const std::uint32_t rp = r & 1u;
-  const std::uint32_t in_base = b.slot_in_base[s];
+  const std::uint32_t in_base = d.nbr_base;
 
   for (std::uint32_t k = tid; k < K; k += B) l.table[k] = g.in_nbr_stage[nb0 + k];
   __syncthreads();
@@ -162,9 +162,9 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
   st.mark(0);
 
   // CSC block of T: internal records [e0, ei), boundary records [ei, e1).
-  const std::uint32_t e0 = g.column_offsets[v0];
-  const std::uint32_t e1 = g.column_offsets[v0 + npt];
-  const std::uint32_t ei = e0 + g.csc_intra[T];
+  const std::uint32_t e0 = d.e0;
+  const std::uint32_t e1 = d.e1;
+  const std::uint32_t ei = d.ei;
   const bool resident = ei - e0 <= static_cast<std::uint32_t>(B * EPT);
   if (tid == 0) {
     // edge records (pack + weight) once, the written column once
@@ -225,7 +225,7 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
   }
   st.mark(2);
 
-  std::uint32_t* out = b.dist + b.slot_dist[s] + static_cast<std::size_t>(rp ^ 1u) * npt;
+  std::uint32_t* out = b.dist + d.dist + static_cast<std::size_t>(rp ^ 1u) * npt;
   bool changed = false;
   for (std::uint32_t i = tid; i < npt; i += B) {
     const std::uint32_t v = l.own[i];
@@ -250,19 +250,19 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
 // ---------------------------------------------------------------------------
 template <int B, int EPT, bool I>
 __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b, std::uint32_t s,
-                           std::uint32_t r, bool frontier, const Lds& l,
+                           const SlotDesc& d, std::uint32_t r, const Lds& l,
                            RecordRegs<B, EPT>& internal, std::uint32_t& regs_key,
                            VisitStats<I>& st) {
   const std::uint32_t tid = threadIdx.x;
   st.start();
-  const std::uint32_t T = b.tiles[s];
-  const std::uint32_t q = b.slot_query[s];
-  const std::uint32_t v0 = g.tile_vertex_offsets[T];
-  const std::uint32_t npt = g.tile_vertex_offsets[T + 1] - v0;
-  const std::uint32_t ob0 = g.out_nbr_offsets[T];
-  const std::uint32_t KO = g.out_nbr_offsets[T + 1] - ob0;
-  const std::uint32_t out_base = b.slot_out_base[s];
-  std::uint32_t* column = b.dist + b.slot_dist[s];
+  const bool frontier = d.engine == kFrontierPush;
+  const std::uint32_t T = d.tile;
+  const std::uint32_t q = d.query;
+  const std::uint32_t npt = d.npt;
+  const std::uint32_t ob0 = d.nbr0;
+  const std::uint32_t KO = d.nbr_count;
+  const std::uint32_t out_base = d.nbr_base;
+  std::uint32_t* column = b.dist + d.dist;
   std::uint32_t* snapshot = column + npt;
 
   const bool proposals = b.boundary_mode == 0;
@@ -272,9 +272,9 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
     l.table[256 + k] = kInfBits;  // smallest label pushed into neighbour k
     if (proposals) {
       l.table[512 + k] = g.out_nbr_stage[ob0 + k];
-    } else if (ns != kNone) {
+    } else {
       // pool offset of the neighbour's labels, as two words
-      const unsigned long long off = b.slot_dist[ns];
+      const unsigned long long off = b.slot_out_off[out_base + k];
       l.table[512 + 2 * k] = static_cast<std::uint32_t>(off);
       l.table[513 + 2 * k] = static_cast<std::uint32_t>(off >> 32);
     }
@@ -300,9 +300,9 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
   st.mark(0);
 
   // CSR block of T: internal records [e0, ei), boundary records [ei, e1).
-  const std::uint32_t e0 = g.row_offsets[v0];
-  const std::uint32_t e1 = g.row_offsets[v0 + npt];
-  const std::uint32_t ei = e0 + g.csr_intra[T];
+  const std::uint32_t e0 = d.e0;
+  const std::uint32_t e1 = d.e1;
+  const std::uint32_t ei = d.ei;
   const bool resident = ei - e0 <= static_cast<std::uint32_t>(B * EPT);
   if (tid == 0) {
     // edge records once, the column (and the snapshot for frontier) once
@@ -456,10 +456,9 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
 // label, to round r + 1.  The slot holding the query's minimum is always
 // visited, so every query advances each round, and any visiting order
 // reaches the same fixed point.  Block-uniform result.
-__device__ __forceinline__ bool in_window(const DevBatch& b, std::uint32_t s, std::uint32_t r,
-                                          const Lds& l) {
+__device__ __forceinline__ bool in_window(const DevBatch& b, std::uint32_t s, std::uint32_t q,
+                                          std::uint32_t r, const Lds& l) {
   if (threadIdx.x == 0) {
-    const std::uint32_t q = b.slot_query[s];
     std::uint32_t* own = b.slot_min + static_cast<std::size_t>(r & 1) * b.slot_count + s;
     const std::uint32_t m = load_agent(own);
     const std::uint32_t lo = load_agent(&b.query_min[static_cast<std::size_t>(r % 3) * b.query_count + q]);
@@ -498,21 +497,21 @@ __device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch&
     const std::uint32_t end = min(start + chunk, n);
     for (std::uint32_t i = start; i < end; ++i) {
       const std::uint32_t s = b.lists[static_cast<std::size_t>(li) * b.slot_count + i];
-      const std::uint32_t engine = b.engine[b.slot_query[s]];
-      if (K == 1 || (K == 3 && engine == kJacobiPull)) {
+      const SlotDesc d = b.slot_desc[s];
+      if (K == 1 || (K == 3 && d.engine == kJacobiPull)) {
         if constexpr (K != 2) {
-          visit_pull<B, EPT, I>(g, b, s, r, l, regs, regs_key, st);
+          visit_pull<B, EPT, I>(g, b, s, d, r, l, regs, regs_key, st);
           ++st.items;
         }
-      } else if (!b.slot_min || in_window(b, s, r, l)) {
+      } else if (!b.slot_min || in_window(b, s, d.query, r, l)) {
         if constexpr (K != 1) {
-          visit_push<B, EPT, I>(g, b, s, r, engine == kFrontierPush, l, regs, regs_key, st);
+          visit_push<B, EPT, I>(g, b, s, d, r, l, regs, regs_key, st);
           ++st.items;
         }
       }
       __syncthreads();
       // the visit's own enqueues are issued before its item is retired
-      if (b.pending && threadIdx.x == 0) atomic_add_agent(&b.pending[b.slot_query[s]], 0xFFFFFFFFu);
+      if (b.pending && threadIdx.x == 0) atomic_add_agent(&b.pending[d.query], 0xFFFFFFFFu);
     }
   }
   if constexpr (I) {

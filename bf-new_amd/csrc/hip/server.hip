@@ -83,6 +83,7 @@ __global__ void admit_kernel(DevGraph g, DevBatch b, const AdmitRecord* records,
     const std::uint32_t o0 = g.out_nbr_offsets[T], o1 = g.out_nbr_offsets[T + 1];
     for (std::uint32_t k = o0; k < o1; ++k)
       b.slot_out[static_cast<std::size_t>(s) * kout + (k - o0)] = find_slot(b, q, g.out_nbr_tiles[k]);
+    fill_slot_desc(g, b, s);
   }
   const unsigned long long w0 = static_cast<unsigned long long>(qb) * slab_words;
   const unsigned long long w1 = static_cast<unsigned long long>(qe) * slab_words;
@@ -223,7 +224,9 @@ struct bfn_server_s {
                 *d_last = nullptr, *d_pending = nullptr, *d_active = nullptr, *d_done = nullptr,
                 *d_result = nullptr, *d_clean = nullptr, *d_summary = nullptr,
                 *d_upload_tiles = nullptr;
-  unsigned long long *d_slot_dist = nullptr, *d_frame_base = nullptr, *d_lengths = nullptr;
+  unsigned long long *d_slot_dist = nullptr, *d_frame_base = nullptr, *d_lengths = nullptr,
+                     *d_out_off = nullptr;
+  SlotDesc* d_desc = nullptr;
   DevControl* d_control = nullptr;
   AdmitRecord* d_records = nullptr;
   // pinned staging
@@ -471,6 +474,7 @@ int32_t bfn_server_create(bfn_device device, bfn_graph graph, const bfn_server_o
             alloc(&sv->d_stamp, 4 * S) && alloc(&sv->d_lists, 12 * S) &&
             alloc(&sv->d_pool, 4 * S * sv->slab_words) && alloc(&sv->d_onpath, 4 * S * sv->slab_words) &&
             alloc(&sv->d_frames, 12 * S * sv->max_v) && alloc(&sv->d_slot_dist, 8 * S) &&
+            alloc(&sv->d_desc, sizeof(SlotDesc) * S) && alloc(&sv->d_out_off, 8 * S * sv->kout) &&
             alloc(&sv->d_begin, 4 * Q) && alloc(&sv->d_end, 4 * Q) && alloc(&sv->d_src_off, 4 * (Q + 1)) &&
             alloc(&sv->d_src, 4 * Q) && alloc(&sv->d_tgt_off, 4 * (Q + 1)) && alloc(&sv->d_tgt, 4 * Q) &&
             alloc(&sv->d_engine, 4 * Q) && alloc(&sv->d_last, 4 * Q) && alloc(&sv->d_pending, 4 * Q) &&
@@ -511,6 +515,8 @@ int32_t bfn_server_create(bfn_device device, bfn_graph graph, const bfn_server_o
   b.engine = sv->d_engine;
   b.slot_query = sv->d_slot_query;
   b.slot_dist = sv->d_slot_dist;
+  b.slot_desc = sv->d_desc;
+  b.slot_out_off = sv->d_out_off;
   b.slot_in_base = sv->d_in_base;
   b.slot_out_base = sv->d_out_base;
   b.slot_in = sv->d_slot_in;
