@@ -53,7 +53,12 @@ constexpr int kPhaseCount = 5;  // visit phases timed by instrumented runs
 // LDS table of a visit: neighbour slots [0, 256), pushed-into marks
 // [256, 512), proposal slab offsets [512, 768); then the flag words.
 constexpr std::uint32_t kLdsTableWords = 1024;
-constexpr std::uint32_t kLdsFlagWords = 8;
+// LDS words after the table: 8 flag words, then (16-byte aligned, hence
+// the 4 spare words) the staged descriptors of one claimed chunk of at most
+// kMaxItemsPerClaim items.
+constexpr std::uint32_t kLdsFlagHeader = 8;
+constexpr std::uint32_t kMaxItemsPerClaim = 8;
+constexpr std::uint32_t kLdsFlagWords = kLdsFlagHeader + 4 + kMaxItemsPerClaim * 16;
 
 enum Engine : std::uint32_t { kJacobiPull = 0, kDensePush = 1, kFrontierPush = 2 };
 

@@ -897,9 +897,9 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
   bfn_tuning t;
   if (tuning) t = *tuning; else bfn_tuning_default(&t);
   if (t.tile_sweeps > 1 || t.boundary_mode > 1 || t.use_graph > 1 || t.min_waves > 8 ||
-      !(t.window >= 0.0f) || !std::isfinite(t.window))
-    return fail(BFN_E_INVALID_ARGUMENT,
-                "invalid tuning (tile_sweeps, boundary_mode, use_graph, min_waves, window)");
+      t.items_per_claim > kMaxItemsPerClaim || !(t.window >= 0.0f) || !std::isfinite(t.window))
+    return fail(BFN_E_INVALID_ARGUMENT, "invalid tuning (tile_sweeps, boundary_mode, use_graph, "
+                                        "min_waves, items_per_claim, window)");
   BatchPlanHost p;
   if (int32_t rc = plan_batch(graph, d, o, p)) return rc;
   BFN_HIP(hipSetDevice(ws->device->ordinal));

@@ -488,16 +488,41 @@ __device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch&
   RecordRegs<B, EPT> regs;
   std::uint32_t regs_key = kNone;  // tile and orientation of the records in regs
   VisitStats<I> st;
+  // Thread 0 keeps the next claim in flight while the current chunk runs;
+  // each chunk's slot ids and descriptors are staged in LDS together (one
+  // dependent load chain per chunk rather than per item).
+  std::uint32_t ahead = 0;
+  if (threadIdx.x == 0) ahead = atomic_add_agent(&b.control->claim[li], chunk);
+  // 16-byte aligned, whatever the word offset of the flags
+  SlotDesc* staged = reinterpret_cast<SlotDesc*>(
+      (reinterpret_cast<std::uintptr_t>(l.flags + kLdsFlagHeader) + 15) & ~std::uintptr_t{15});
   for (;;) {
-    if (threadIdx.x == 0) l.flags[0] = atomic_add_agent(&b.control->claim[li], chunk);
+    if (threadIdx.x == 0) {
+      l.flags[0] = ahead;
+      if (ahead < n) ahead = atomic_add_agent(&b.control->claim[li], chunk);
+    }
     __syncthreads();
     const std::uint32_t start = l.flags[0];
-    __syncthreads();
     if (start >= n) break;
-    const std::uint32_t end = min(start + chunk, n);
-    for (std::uint32_t i = start; i < end; ++i) {
-      const std::uint32_t s = b.lists[static_cast<std::size_t>(li) * b.slot_count + i];
-      const SlotDesc d = b.slot_desc[s];
+    const std::uint32_t count = min(start + chunk, n) - start;
+    constexpr std::uint32_t kWords = sizeof(SlotDesc) / 4;
+    for (std::uint32_t t = threadIdx.x; t < count * kWords; t += B) {
+      const std::uint32_t s = b.lists[static_cast<std::size_t>(li) * b.slot_count + start + t / kWords];
+      reinterpret_cast<std::uint32_t*>(staged)[t] =
+          reinterpret_cast<const std::uint32_t*>(b.slot_desc + s)[t % kWords];
+      if (t % kWords == kWords - 1) reinterpret_cast<std::uint32_t*>(staged)[t] = s;  // pad2 high word
+    }
+    __syncthreads();
+    for (std::uint32_t i = 0; i < count; ++i) {
+      // uniform: kept in scalar registers
+      SlotDesc d;
+      {
+        const std::uint32_t* w = reinterpret_cast<const std::uint32_t*>(staged + i);
+        std::uint32_t* o = reinterpret_cast<std::uint32_t*>(&d);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) o[k] = __builtin_amdgcn_readfirstlane(w[k]);
+      }
+      const std::uint32_t s = static_cast<std::uint32_t>(d.pad2 >> 32);
       if (K == 1 || (K == 3 && d.engine == kJacobiPull)) {
         if constexpr (K != 2) {
           visit_pull<B, EPT, I>(g, b, s, d, r, l, regs, regs_key, st);

@@ -126,7 +126,8 @@ typedef struct {
   uint32_t tile_sweeps;          /* Jacobi: 1 = point Jacobi (reference round
                                     semantics), 0 = sweep each tile to its local
                                     fixed point every round (block Jacobi) */
-  uint32_t items_per_claim;      /* work items taken per queue claim */
+  uint32_t items_per_claim;      /* work items taken per queue claim (0-8;
+                                    0 means 1) */
   uint32_t graph_rounds;         /* rounds captured per hipGraph replay */
   uint32_t use_graph;            /* replay the batch step through a hipGraph */
   uint32_t count_relaxations;    /* compute per-query relaxation counts */
