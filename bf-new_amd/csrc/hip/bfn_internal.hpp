@@ -103,6 +103,7 @@ struct DevGraph {
   const uint2* csr_rec;
   const std::uint32_t* csc_intra;
   const std::uint32_t* csr_intra;
+  const std::uint32_t* csr_bnd;  // [V] first boundary csr_rec record of each vertex
 };
 
 // Control block; lives in device memory, one per workspace.

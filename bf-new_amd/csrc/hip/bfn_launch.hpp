@@ -75,6 +75,7 @@ struct TileImageArgs {
   std::uint32_t* csr_perm;
   std::uint32_t* csc_intra;
   std::uint32_t* csr_intra;
+  std::uint32_t* csr_bnd;
   std::uint32_t* stats;
   std::uint32_t* host_in_total;
   std::uint32_t* host_out_total;

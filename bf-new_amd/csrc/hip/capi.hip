@@ -126,7 +126,7 @@ struct bfn_graph_s {
   std::uint32_t V = 0, E = 0, T = 0, Rcsr = 0, Rcsc = 0;
   DeviceBuffer owner, tile_off, row_off, dst, w, row_runs, run_edges, run_dst, col_off, src, cw,
       eid, col_runs, crun_edges, crun_src, in_off, out_off, csc_pack, csr_pack, csc_rec, csr_rec,
-      csc_perm, csr_perm, csc_intra, csr_intra, base_cost, occ_scratch, counts, stats;
+      csc_perm, csr_perm, csc_intra, csr_intra, csr_bnd, base_cost, occ_scratch, counts, stats;
   std::uint32_t* in_tiles = nullptr;
   std::uint32_t* in_stage = nullptr;
   std::uint32_t* out_tiles = nullptr;
@@ -336,6 +336,7 @@ int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_gra
   if (e == hipSuccess) e = g->csr_perm.reserve(sizeof(std::uint32_t) * e_words);
   if (e == hipSuccess) e = g->csc_intra.reserve(sizeof(std::uint32_t) * (T + 1));
   if (e == hipSuccess) e = g->csr_intra.reserve(sizeof(std::uint32_t) * (T + 1));
+  if (e == hipSuccess) e = g->csr_bnd.reserve(sizeof(std::uint32_t) * std::max(V, 1u));
   if (e == hipSuccess) e = g->stats.reserve(sizeof(std::uint32_t) * 8);
   if (e == hipSuccess) e = hipMemsetAsync(g->stats.ptr, 0, sizeof(std::uint32_t) * 8, s);
   if (e != hipSuccess) return cleanup(BFN_E_HIP, std::string("graph upload: ") + hipGetErrorString(e));
@@ -370,6 +371,7 @@ int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_gra
   ta.csr_perm = g->csr_perm.as<std::uint32_t>();
   ta.csc_intra = g->csc_intra.as<std::uint32_t>();
   ta.csr_intra = g->csr_intra.as<std::uint32_t>();
+  ta.csr_bnd = g->csr_bnd.as<std::uint32_t>();
   ta.stats = g->stats.as<std::uint32_t>();
   ta.host_in_total = &in_total;
   ta.host_out_total = &out_total;
@@ -420,6 +422,7 @@ int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_gra
   v.csr_rec = g->csr_rec.as<uint2>();
   v.csc_intra = g->csc_intra.as<std::uint32_t>();
   v.csr_intra = g->csr_intra.as<std::uint32_t>();
+  v.csr_bnd = g->csr_bnd.as<std::uint32_t>();
 
   bfn_graph_info& info = g->info;
   info.max_tile_vertices = max_v;
@@ -434,8 +437,8 @@ int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_gra
   info.max_stage_floats = stats[1];
   info.has_zero_weight = stats[4];
   // reference image + derived tile lists + packs (4 B) + records (8 B) +
-  // record permutations (4 B) per edge and orientation
-  info.device_bytes = 4ull * (2ull * V + 2 + T + 1 + 6ull * E + 2ull * (g->Rcsr + g->Rcsc + 2) +
+  // record permutations (4 B) per edge and orientation + boundary row starts
+  info.device_bytes = 4ull * (3ull * V + 2 + T + 1 + 6ull * E + 2ull * (g->Rcsr + g->Rcsc + 2) +
                               4ull * (T + 1) + 2ull * in_total + 2ull * out_total) +
                       2ull * E * (4 + 8 + 4);
   *out = g;
@@ -974,8 +977,10 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
                          lds_words(ws->max_v, gv.max_tile_vertices + gv.max_out_stage_floats) * 4ull <=
                              lds_limit;
   if (!proposals) ws->view.boundary_mode = 1;
+  // push, direct boundary: snapshot + per-vertex boundary starts (npt + 1)
   ws->work_words = std::max({p.any_jacobi ? gv.max_stage_floats : 0u, gv.max_tile_vertices,
-                             proposals ? gv.max_tile_vertices + gv.max_out_stage_floats : 0u});
+                             proposals ? gv.max_tile_vertices + gv.max_out_stage_floats : 0u,
+                             p.any_push && !proposals ? 2 * gv.max_tile_vertices + 1 : 0u});
   ws->lds_bytes = 4ull * lds_words(ws->max_v, ws->work_words);
   if (ws->lds_bytes > lds_limit)
     return fail(BFN_E_NOT_REPRESENTABLE, "tile neighbourhood exceeds the LDS budget (" +

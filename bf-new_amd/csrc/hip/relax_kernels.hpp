@@ -285,7 +285,10 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
     l.orig[i] = x;
     // Dense pushes every finite vertex: an infinite snapshot admits them all.
     l.work[i] = frontier ? snapshot[i] : kInfBits;
+    // direct mode: where each vertex's boundary records start
+    if (!proposals) l.prop[i] = g.csr_bnd[d.v0 + i];
   }
+  if (!proposals && tid == 0) l.prop[npt] = d.e1;
   if (proposals) {
     // Clear the proposal slab of every selected out-neighbour (k >= 1).
     for (std::uint32_t k = 1; k < KO; ++k) {
@@ -305,9 +308,12 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
   const std::uint32_t ei = d.ei;
   const bool resident = ei - e0 <= static_cast<std::uint32_t>(B * EPT);
   if (tid == 0) {
-    // edge records once, the column (and the snapshot for frontier) once
-    st.bytes += 8ull * (e1 - e0) + 4ull * npt * (frontier ? 2 : 1);
-    st.edges += e1 - e0;
+    // internal records once, the column (and the snapshot for frontier) once;
+    // boundary records: all of them (proposals), or those of the pushing
+    // vertices plus the per-vertex boundary starts (direct, counted below)
+    st.bytes += 8ull * ((proposals ? e1 : ei) - e0) + 4ull * npt * (frontier ? 2 : 1) +
+                (proposals ? 0ull : 4ull * npt);
+    st.edges += (proposals ? e1 : ei) - e0;
   }
   // The internal records stay in registers from a previous visit of T.
   const std::uint32_t key = 2 * T + 1;
@@ -394,27 +400,54 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
       target = b.dist + off + pk_dst(p);
       return as_bits(as_float(sv) + w);
     };
-    for (std::uint32_t base = ei; base < e1; base += B * stream_records(B)) {
-      RecordRegs<B, stream_records(B)> chunk;
-      load_records<B, stream_records(B)>(chunk, g.csr_rec, base, e1);
-      std::uint32_t seen[stream_records(B)];
+    // Only the boundary records of vertices that push (frontier: label below
+    // the snapshot; dense: finite) are read: each lane walks its own
+    // vertices' record ranges, R records at a time.
+    constexpr int R = stream_records(B);
+    std::uint32_t vi = tid;             // next vertex of this lane
+    std::uint32_t e = 0, e_end = 0;     // the lane's current record range
+    unsigned long long read = 0;
+    for (;;) {
+      std::uint32_t idx[R];
 #pragma unroll
-      for (int j = 0; j < stream_records(B); ++j) {
+      for (int j = 0; j < R; ++j) {
+        while (e == e_end && vi < npt) {
+          if (l.own[vi] < l.work[vi]) {
+            e = l.prop[vi];
+            e_end = l.prop[vi + 1];
+          }
+          vi += B;
+        }
+        idx[j] = e < e_end ? e++ : kNone;
+      }
+      if (idx[0] == kNone) break;
+      uint2 rec[R];
+#pragma unroll
+      for (int j = 0; j < R; ++j) rec[j] = idx[j] != kNone ? g.csr_rec[idx[j]] : make_uint2(kNone, 0u);
+      std::uint32_t seen[R];
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
         std::uint32_t* target = nullptr;
-        seen[j] = candidate(chunk.pack[j], chunk.w[j], target) == kNone ? 0u : *target;
+        seen[j] = candidate(rec[j].x, __uint_as_float(rec[j].y), target) == kNone ? 0u : *target;
       }
 #pragma unroll
-      for (int j = 0; j < stream_records(B); ++j) {
+      for (int j = 0; j < R; ++j) {
+        if (idx[j] == kNone) continue;
+        ++read;
         std::uint32_t* target = nullptr;
-        const std::uint32_t cb = candidate(chunk.pack[j], chunk.w[j], target);
+        const std::uint32_t cb = candidate(rec[j].x, __uint_as_float(rec[j].y), target);
         if (cb == kNone) continue;
         ++att;
         if (cb < seen[j]) {
           atomic_min_agent(target, cb);
           ++suc;
-          atomicMin(&l.table[256 + pk_k(chunk.pack[j])], cb);
+          atomicMin(&l.table[256 + pk_k(rec[j].x)], cb);
         }
       }
+    }
+    if constexpr (I) {
+      st.bytes += 8ull * read;
+      st.edges += read;
     }
   }
   st.attempts += att;

@@ -441,7 +441,9 @@ int32_t bfn_server_create(bfn_device device, bfn_graph graph, const bfn_server_o
   sv->slab_words = 2 * sv->max_v;
   // Push engines push boundary edges with direct atomics here (no proposal
   // area), which keeps more workgroups resident for small requests.
-  sv->work_words = std::max(o->engine == kJacobiPull ? sv->gview.max_stage_floats : 0u, sv->max_v);
+  // push: snapshot + per-vertex boundary starts (npt + 1)
+  sv->work_words = o->engine == kJacobiPull ? std::max(sv->gview.max_stage_floats, sv->max_v)
+                                            : 2 * sv->max_v + 1;
   sv->lds_bytes = 4ull * (2ull * sv->max_v + sv->work_words + kLdsTableWords + kLdsFlagWords);
   if (sv->lds_bytes > device_lds_limit(device))
     return bail(BFN_E_NOT_REPRESENTABLE, "tile neighbourhood exceeds the LDS budget");
