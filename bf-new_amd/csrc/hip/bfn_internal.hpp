@@ -59,6 +59,15 @@ constexpr std::uint32_t kLdsTableWords = 1024;
 constexpr std::uint32_t kLdsFlagHeader = 8;
 constexpr std::uint32_t kMaxItemsPerClaim = 8;
 constexpr std::uint32_t kLdsFlagWords = kLdsFlagHeader + 4 + kMaxItemsPerClaim * 16;
+// Then the (slot, smallest pushed label) pairs of the push visits of one
+// round, queued for the next round when the round's work is done.
+constexpr std::uint32_t kLdsDeferEntries = 256;
+constexpr std::uint32_t kLdsDeferWords = 2 * kLdsDeferEntries;
+
+// Dynamic LDS words of a relaxation workgroup.
+constexpr std::size_t relax_lds_words(std::uint32_t max_v, std::uint32_t work_words) {
+  return 2ull * max_v + work_words + kLdsTableWords + kLdsFlagWords + kLdsDeferWords;
+}
 
 enum Engine : std::uint32_t { kJacobiPull = 0, kDensePush = 1, kFrontierPush = 2 };
 

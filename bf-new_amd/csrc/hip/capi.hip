@@ -105,7 +105,7 @@ constexpr std::size_t align_up(std::size_t x, std::size_t a) { return (x + a - 1
 // LDS words of one relaxation workgroup: own + orig labels, the work area
 // (staged slabs, or snapshot + proposals), the table and the flags.
 constexpr std::size_t lds_words(std::uint32_t max_v, std::uint32_t work_words) {
-  return 2ull * max_v + work_words + kLdsTableWords + kLdsFlagWords;
+  return relax_lds_words(max_v, work_words);
 }
 
 bool valid_block(std::uint32_t b) {

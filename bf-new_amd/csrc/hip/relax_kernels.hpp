@@ -48,6 +48,7 @@ struct Lds {
   std::uint32_t* prop;
   std::uint32_t* table;
   std::uint32_t* flags;
+  std::uint32_t* defer;   // deferred (slot, label) enqueues of this round
 };
 
 __device__ __forceinline__ Lds carve(std::uint32_t* base, std::uint32_t max_v,
@@ -59,6 +60,7 @@ __device__ __forceinline__ Lds carve(std::uint32_t* base, std::uint32_t max_v,
   l.prop = l.work + max_v;
   l.table = l.work + work_words;
   l.flags = l.table + kLdsTableWords;
+  l.defer = l.flags + kLdsFlagWords;
   return l;
 }
 
@@ -469,7 +471,16 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
   bool marked = false;
   for (std::uint32_t k = 1 + tid; k < KO; k += B) {
     if (l.table[256 + k] != kInfBits) {
-      enqueue_pushed(b, l.table[k], r + 1, l.table[256 + k]);
+      // Queued when the round's work is done (one enqueue chain per
+      // workgroup and round, not per visit); the serving loop, which counts
+      // pending work per visit, and a full buffer enqueue at once.
+      const std::uint32_t pos = b.pending ? kLdsDeferEntries : atomicAdd(&l.flags[2], 1u);
+      if (pos < kLdsDeferEntries) {
+        l.defer[2 * pos] = l.table[k];
+        l.defer[2 * pos + 1] = l.table[256 + k];
+      } else {
+        enqueue_pushed(b, l.table[k], r + 1, l.table[256 + k]);
+      }
       marked = true;
     }
   }
@@ -525,7 +536,10 @@ __device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch&
   // each chunk's slot ids and descriptors are staged in LDS together (one
   // dependent load chain per chunk rather than per item).
   std::uint32_t ahead = 0;
-  if (threadIdx.x == 0) ahead = atomic_add_agent(&b.control->claim[li], chunk);
+  if (threadIdx.x == 0) {
+    ahead = atomic_add_agent(&b.control->claim[li], chunk);
+    l.flags[2] = 0;  // deferred enqueues
+  }
   // 16-byte aligned, whatever the word offset of the flags
   SlotDesc* staged = reinterpret_cast<SlotDesc*>(
       (reinterpret_cast<std::uintptr_t>(l.flags + kLdsFlagHeader) + 15) & ~std::uintptr_t{15});
@@ -572,6 +586,11 @@ __device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch&
       if (b.pending && threadIdx.x == 0) atomic_add_agent(&b.pending[d.query], 0xFFFFFFFFu);
     }
   }
+  // The round's deferred enqueues (start reads flags[0] after the last
+  // barrier above, so every visit's pairs are in place).
+  const std::uint32_t deferred = min(l.flags[2], kLdsDeferEntries);
+  for (std::uint32_t i = threadIdx.x; i < deferred; i += B)
+    enqueue_pushed(b, l.defer[2 * i], r + 1, l.defer[2 * i + 1]);
   if constexpr (I) {
     // One add per thread keeps the counters off the hot path's critical lanes.
     DevControl* c = b.control;

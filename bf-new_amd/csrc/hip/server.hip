@@ -444,7 +444,7 @@ int32_t bfn_server_create(bfn_device device, bfn_graph graph, const bfn_server_o
   // push: snapshot + per-vertex boundary starts (npt + 1)
   sv->work_words = o->engine == kJacobiPull ? std::max(sv->gview.max_stage_floats, sv->max_v)
                                             : 2 * sv->max_v + 1;
-  sv->lds_bytes = 4ull * (2ull * sv->max_v + sv->work_words + kLdsTableWords + kLdsFlagWords);
+  sv->lds_bytes = 4ull * relax_lds_words(sv->max_v, sv->work_words);
   if (sv->lds_bytes > device_lds_limit(device))
     return bail(BFN_E_NOT_REPRESENTABLE, "tile neighbourhood exceeds the LDS budget");
   sv->block = o->block_size;

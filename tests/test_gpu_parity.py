@@ -298,12 +298,20 @@ def test_c3_batch_certificate(device):
     windowed = run(device, dg, batch, bfn.FRONTIER_PUSH, bfn.CHUNKED, 0, block=256,
                    items_per_claim=2, use_graph=1, graph_rounds=48, boundary_mode=1, window=2.0)
     assert windowed.status()[0].converged == 1
+    # One workgroup per CU: each workgroup visits ~100 tiles per round, more
+    # than its LDS holds of deferred next-round enqueues (the overflow path).
+    narrow = bfn.Workspace(device)
+    narrow.submit(dg, batch, bfn.run_options(engine=bfn.DENSE_CHAOTIC_PUSH, control_mode=bfn.CHUNKED,
+                                             block_size=256, grid_policy=1, blocks_per_wgp=1),
+                  bfn.tuning(tile_sweeps=0, boundary_mode=1))
+    assert narrow.status()[0].converged == 1
     sample = list(range(0, 256, 16)) + [i for i, q in enumerate(qs) if q.unbounded][:2]
     for i in sample:
         d = ws.distances(i)
         assert certify_sssp(lay, [qs[i].source], tiles[i], d) == "", i
         assert np.array_equal(bits(tuned.distances(i)), bits(d)), i
         assert np.array_equal(bits(windowed.distances(i)), bits(d)), i
+        assert np.array_equal(bits(narrow.distances(i)), bits(d)), i
     assert [p["distance"] for p in tuned.paths()] == [p["distance"] for p in ws.paths()]
     view = dict(vertex_count=lay["vertex_count"], tile_count=lay["tile_count"],
                 row_offsets=lay["csr_row_offsets"].astype(np.uint64),
