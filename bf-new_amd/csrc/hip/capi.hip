@@ -991,14 +991,6 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
                           ws->view.engine_kinds,
                           ws->lds_bytes, &per_cu));
   if (per_cu <= 0) return fail(BFN_E_NOT_REPRESENTABLE, "relaxation kernel cannot be resident");
-  if (o->control_mode == 0) {
-    // A persistent grid must be fully resident (it meets at a grid barrier).
-    // The occupancy query can count 8 waves per SIMD for a kernel whose
-    // vector registers allow 8 while its scalar registers allow only 7 (a
-    // SIMD holds 800 SGPRs, a wave up to 112 with VCC and the trap
-    // registers), so persistent grids stay within 7 waves per SIMD.
-    per_cu = std::min(per_cu, std::max(1, static_cast<int>(7u * 4u * 64u / ws->block)));
-  }
   const std::uint32_t cus = static_cast<std::uint32_t>(ws->device->props.multiProcessorCount);
   std::uint32_t blocks = static_cast<std::uint32_t>(per_cu) * cus;
   if (o->grid_policy == 1) blocks = std::min(blocks, o->blocks_per_wgp * cus);
