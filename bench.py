@@ -67,7 +67,7 @@ def profiled_traffic(workload, tune):
     if not entry:
         return None, None
     cfg = entry["config"]
-    keys = ("engine", "block", "sweeps", "boundary_mode", "use_graph", "min_waves", "window")
+    keys = ("engine", "block", "sweeps", "boundary_mode", "use_graph", "min_waves")
     if any(float(cfg.get(k, 0)) != float(tune.get(k, 0)) for k in keys):
         return None, None
     return entry["hbm_bytes_per_step"], entry["source"]
@@ -89,7 +89,7 @@ def make_batch(hg, cfg):
 def apply_tuning(batch, qs, tune_entry):
     t = dict(engine=bfn.JACOBI_PULL, block=256, sweeps=0, control=bfn.PERSISTENT,
              classes=None, items_per_claim=1, use_graph=0, graph_rounds=0, boundary_mode=0,
-             min_waves=0, window=0.0)
+             min_waves=0)
     if tune_entry:
         t.update(tune_entry)
     if t.get("classes"):
@@ -286,7 +286,7 @@ def negotiated(dev, tune):
         ws.submit(dg, batch, opts, bfn.tuning(tile_sweeps=tune["sweeps"], count_relaxations=1,
                                               items_per_claim=tune["items_per_claim"],
                                               boundary_mode=tune["boundary_mode"],
-                                              min_waves=tune["min_waves"], window=tune["window"]))
+                                              min_waves=tune["min_waves"]))
         st, per, _ = ws.status()
         relax_ms = ws.timing()[0]
         relaxations = sum(p.relaxations for p in per)
@@ -367,7 +367,7 @@ def main():
               bfn.tuning(tile_sweeps=tune["sweeps"], count_relaxations=1,
                          items_per_claim=tune["items_per_claim"], use_graph=tune["use_graph"],
                          graph_rounds=tune["graph_rounds"],
-                         boundary_mode=tune["boundary_mode"], min_waves=tune["min_waves"], window=tune["window"]))
+                         boundary_mode=tune["boundary_mode"], min_waves=tune["min_waves"]))
     st, per, _ = ws.status()
     if not st.converged:
         raise SystemExit("batch did not converge")
@@ -379,7 +379,7 @@ def main():
                                           use_graph=tune["use_graph"],
                                           graph_rounds=tune["graph_rounds"],
                                           boundary_mode=tune["boundary_mode"],
-                                          min_waves=tune["min_waves"], window=tune["window"]))
+                                          min_waves=tune["min_waves"]))
     elapsed, relax_ms, launches = timed_steps(ws, args.steps, args.warmup)
     # one round kernel + one advance kernel per host-polled or replayed round
     round_launches = 1 if tune["control"] == bfn.PERSISTENT and not tune["use_graph"] else launches // 2
@@ -419,7 +419,7 @@ def main():
                    "unbounded_fraction": C3["unbounded"], "engine": bfn.ENGINE_NAMES[tune["engine"]],
                    "per_class_engines": tune.get("classes"), "block": tune["block"],
                    "tile_sweeps": tune["sweeps"], "boundary_mode": tune["boundary_mode"],
-                   "min_waves": tune["min_waves"], "window": tune["window"],
+                   "min_waves": tune["min_waves"],
                    "control": bfn.CONTROL_NAMES[tune["control"]] +
                    (f"+hipgraph({tune['graph_rounds']} rounds)" if tune["use_graph"] else ""),
                    "relaxations_per_step": relax_per_step},

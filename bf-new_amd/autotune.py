@@ -7,8 +7,7 @@ uses graph control, a persistent cooperative launch otherwise) and keeps the
 fastest.  Each candidate must reproduce the first candidate's target distances
 bit for bit, otherwise the tuner stops with an error: every schedule of a
 monotone fp32 relaxation reaches the same fixed point, so a difference is a
-defect, never a tuning trade-off.  The label window of the push engines is
-then swept at the fastest push shape.
+defect, never a tuning trade-off.
 
 Per-class choice: the queries are split into small / medium / large classes by
 box side, the engines are timed on each class alone, and the mixed assignment
@@ -68,8 +67,7 @@ def time_candidate(dev, dg, batch, cand, reps):
                           use_graph=cand.get("use_graph", 0),
                           graph_rounds=cand.get("graph_rounds", 0),
                           boundary_mode=cand.get("boundary_mode", 0),
-                          min_waves=cand.get("min_waves", 0),
-                          window=cand.get("window", 0.0))
+                          min_waves=cand.get("min_waves", 0))
         ws.submit(dg, batch, opts, tune)
         st, _, _ = ws.status()
         if not st.converged:
@@ -110,8 +108,6 @@ def main():
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--blocks", default="128,256,512")
     ap.add_argument("--quick", action="store_true")
-    ap.add_argument("--windows", default="0.5,1,2,4,8",
-                    help="label windows swept at the fastest push shape ('' = none)")
     ap.add_argument("--out", default=os.path.join(HERE, "tuning"))
     args = ap.parse_args()
     cfg = WORKLOADS[args.workload]
@@ -144,7 +140,7 @@ def main():
         trials.append(dict(cand, ms=ms, rounds=rounds, scope="all"))
         log(f"{bfn.ENGINE_NAMES[cand['engine']]:>20} block {cand['block']:4d} sweeps "
             f"{cand['sweeps']} boundary {cand['boundary_mode']} waves {cand['min_waves']} "
-            f"window {cand.get('window', 0.0)} ipc {cand['items_per_claim']} control "
+            f"ipc {cand['items_per_claim']} control "
             f"{bfn.CONTROL_NAMES[cand['control']]}{'+graph' if cand.get('use_graph') else ''}: "
             f"{ms:.3f} ms, {rounds} rounds")
         return rounds
@@ -159,16 +155,6 @@ def main():
 
     for cand in candidates(engines, blocks, args.quick):
         with_graph_twin(cand)
-    # Label windows at the fastest push shape.
-    push = [t for t in trials if t["engine"] != bfn.JACOBI_PULL]
-    windows = [float(w) for w in args.windows.split(",") if w.strip()]
-    if push and windows:
-        shape = min(push, key=lambda t: t["ms"])
-        for w in windows:
-            with_graph_twin(dict({k: v for k, v in shape.items()
-                                  if k not in ("ms", "rounds", "scope", "use_graph",
-                                               "graph_rounds")},
-                                 control=bfn.PERSISTENT, window=w))
     best = min(trials, key=lambda t: t["ms"])
 
     # Per-class engines at the best uniform shape.
@@ -183,8 +169,7 @@ def main():
         for engine in engines:
             cand = dict(best, engine=engine, sweeps=best["sweeps"] if engine == 0 else 0,
                         boundary_mode=0 if engine == 0 else best.get("boundary_mode", 0),
-                        min_waves=0 if engine == 0 else best.get("min_waves", 0),
-                        window=0.0 if engine == 0 else best.get("window", 0.0))
+                        min_waves=0 if engine == 0 else best.get("min_waves", 0))
             ms, _, rounds = time_candidate(dev, dg, sub, cand, args.reps)
             if ms is not None:
                 timing[engine] = ms
@@ -194,7 +179,6 @@ def main():
             {bfn.ENGINE_NAMES[e]: round(v, 3) for e, v in timing.items()})
     entry = {k: best[k] for k in ("engine", "block", "sweeps", "control", "items_per_claim",
                                   "boundary_mode", "min_waves")}
-    entry["window"] = best.get("window", 0.0)
     entry["use_graph"] = best.get("use_graph", 0)
     entry["graph_rounds"] = best.get("graph_rounds", 0)
     entry["ms"] = best["ms"]
@@ -221,8 +205,7 @@ def main():
     with open(table_path, "w") as f:
         json.dump(table, f, indent=1, sort_keys=True)
     with open(os.path.join(args.out, f"trials_{args.workload}.tsv"), "w") as f:
-        cols = ["scope", "engine", "block", "sweeps", "boundary_mode", "min_waves", "window",
-                "items_per_claim",
+        cols = ["scope", "engine", "block", "sweeps", "boundary_mode", "min_waves", "items_per_claim",
                 "control", "use_graph", "graph_rounds", "rounds", "ms"]
         f.write("\t".join(cols) + "\n")
         for t in trials:

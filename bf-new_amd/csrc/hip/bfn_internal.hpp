@@ -53,21 +53,7 @@ constexpr int kPhaseCount = 5;  // visit phases timed by instrumented runs
 // LDS table of a visit: neighbour slots [0, 256), pushed-into marks
 // [256, 512), proposal slab offsets [512, 768); then the flag words.
 constexpr std::uint32_t kLdsTableWords = 1024;
-// LDS words after the table: 8 flag words, then (16-byte aligned, hence
-// the 4 spare words) the staged descriptors of one claimed chunk of at most
-// kMaxItemsPerClaim items.
-constexpr std::uint32_t kLdsFlagHeader = 8;
-constexpr std::uint32_t kMaxItemsPerClaim = 8;
-constexpr std::uint32_t kLdsFlagWords = kLdsFlagHeader + 4 + kMaxItemsPerClaim * 16;
-// Then the (slot, smallest pushed label) pairs of the push visits of one
-// round, queued for the next round when the round's work is done.
-constexpr std::uint32_t kLdsDeferEntries = 256;
-constexpr std::uint32_t kLdsDeferWords = 2 * kLdsDeferEntries;
-
-// Dynamic LDS words of a relaxation workgroup.
-constexpr std::size_t relax_lds_words(std::uint32_t max_v, std::uint32_t work_words) {
-  return 2ull * max_v + work_words + kLdsTableWords + kLdsFlagWords + kLdsDeferWords;
-}
+constexpr std::uint32_t kLdsFlagWords = 8;
 
 enum Engine : std::uint32_t { kJacobiPull = 0, kDensePush = 1, kFrontierPush = 2 };
 
@@ -112,7 +98,6 @@ struct DevGraph {
   const uint2* csr_rec;
   const std::uint32_t* csc_intra;
   const std::uint32_t* csr_intra;
-  const std::uint32_t* csr_bnd;  // [V] first boundary csr_rec record of each vertex
 };
 
 // Control block; lives in device memory, one per workspace.
@@ -138,17 +123,6 @@ struct DevControl {
   unsigned long long phase_cycles[kPhaseCount];  // summed over workgroups
 };
 
-// What a visit needs before its first label load, per slot, in one record:
-// the push engines' out-neighbour orientation, Jacobi's in-neighbour one.
-struct SlotDesc {
-  std::uint32_t tile, query, engine, v0;
-  std::uint32_t npt, nbr0, nbr_count, nbr_base;  // neighbour-tile list of the tile, slot's list
-  std::uint32_t e0, ei, e1, pad;                 // records: internal [e0, ei), boundary [ei, e1)
-  unsigned long long dist;                       // pool offset of the slot's labels
-  unsigned long long pad2;
-};
-static_assert(sizeof(SlotDesc) == 64, "one 64-byte record per slot");
-
 struct DevBatch {
   std::uint32_t query_count;
   std::uint32_t slot_count;
@@ -158,8 +132,6 @@ struct DevBatch {
   std::uint32_t instrumentation;
   std::uint32_t boundary_mode;    // push: 0 LDS proposals when they fit, 1 direct atomics
   std::uint32_t min_waves;        // kernel variant bounded to >= this many waves per SIMD
-  float window;                   // push: label window of eligible slots, 0 = all queued
-  std::uint32_t engine_kinds;     // engines present: 1 Jacobi, 2 push, 3 both
   const std::uint32_t* tile_begin;      // [Q]   first position of q in query_tiles
   const std::uint32_t* tile_end;        // [Q]   one past q's last position
   const std::uint32_t* query_tiles;     // [.]   q's selected tiles, sorted
@@ -176,18 +148,11 @@ struct DevBatch {
   const std::uint32_t* slot_out_base;   // [S]
   std::uint32_t* slot_in;
   std::uint32_t* slot_out;
-  SlotDesc* slot_desc;                  // [S]
-  unsigned long long* slot_out_off;     // pool offset of each slot_out entry, ~0 for none
   std::uint32_t* stamp;                 // [S]
   std::uint32_t* lists;                 // [3*S]
   std::uint32_t* dist;                  // float bits pool
   std::uint32_t* last_change;           // [Q] 1 + last round with a decrease
   std::uint32_t* pending;               // [Q] queued work items of q, or null
-  // Label window (push engines, window > 0): smallest label pushed into each
-  // slot for round r, slot_min[(r & 1) * S + s], and the smallest of those
-  // over q's queued slots, query_min[(r % 3) * Q + q].  Null when unused.
-  std::uint32_t* slot_min;              // [2*S]
-  std::uint32_t* query_min;             // [3*Q]
   DevControl* control;
 };
 
@@ -261,50 +226,6 @@ __device__ __forceinline__ void enqueue(const DevBatch& b, std::uint32_t slot,
     const std::uint32_t pos = atomic_add_agent(&b.control->list_count[li], 1u);
     b.lists[static_cast<std::size_t>(li) * b.slot_count + pos] = slot;
     if (b.pending) atomic_add_agent(&b.pending[b.slot_query[slot]], 1u);
-  }
-}
-
-// Queues slot for `round` with the smallest label pushed into it (the label
-// window of the push engines orders slots by it).
-__device__ __forceinline__ void enqueue_pushed(const DevBatch& b, std::uint32_t slot,
-                                               std::uint32_t round, std::uint32_t label) {
-  if (b.slot_min) {
-    atomic_min_agent(&b.slot_min[static_cast<std::size_t>(round & 1) * b.slot_count + slot], label);
-    atomic_min_agent(&b.query_min[static_cast<std::size_t>(round % 3) * b.query_count +
-                                  b.slot_query[slot]],
-                     label);
-  }
-  enqueue(b, slot, round);
-}
-
-// Writes slot s's descriptor and the pool offsets of its out-neighbour
-// slots (slot_out must be resolved).
-__device__ __forceinline__ void fill_slot_desc(const DevGraph& g, const DevBatch& b,
-                                               std::uint32_t s) {
-  SlotDesc d;
-  d.tile = b.tiles[s];
-  d.query = b.slot_query[s];
-  d.engine = b.engine[d.query];
-  d.v0 = g.tile_vertex_offsets[d.tile];
-  d.npt = g.tile_vertex_offsets[d.tile + 1] - d.v0;
-  const bool pull = d.engine == kJacobiPull;
-  const std::uint32_t* nbr = pull ? g.in_nbr_offsets : g.out_nbr_offsets;
-  d.nbr0 = nbr[d.tile];
-  d.nbr_count = nbr[d.tile + 1] - d.nbr0;
-  d.nbr_base = pull ? b.slot_in_base[s] : b.slot_out_base[s];
-  const std::uint32_t* offsets = pull ? g.column_offsets : g.row_offsets;
-  d.e0 = offsets[d.v0];
-  d.ei = d.e0 + (pull ? g.csc_intra : g.csr_intra)[d.tile];
-  d.e1 = offsets[d.v0 + d.npt];
-  d.pad = 0;
-  d.dist = b.slot_dist[s];
-  d.pad2 = 0;
-  b.slot_desc[s] = d;
-  const std::uint32_t ob = b.slot_out_base[s];
-  const std::uint32_t ko = g.out_nbr_offsets[d.tile + 1] - g.out_nbr_offsets[d.tile];
-  for (std::uint32_t k = 0; k < ko; ++k) {
-    const std::uint32_t ns = b.slot_out[ob + k];
-    b.slot_out_off[ob + k] = ns == kNone ? ~0ull : b.slot_dist[ns];
   }
 }
 

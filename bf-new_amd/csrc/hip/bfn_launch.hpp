@@ -19,30 +19,12 @@ void graph_view(bfn_graph graph, DevGraph* view, std::uint32_t* max_in, std::uin
 
 struct PathArgs;
 
-// Relaxation rounds of a batch: the kernel variant follows the block size,
-// b.instrumentation, b.engine_kinds and b.min_waves.
 hipError_t relax_launch(std::uint32_t block, bool persistent, bool cooperative,
                         std::uint32_t grid, std::size_t lds_bytes, std::uint32_t max_v,
                         std::uint32_t work_words, const DevGraph& g, const DevBatch& b,
                         hipStream_t stream);
-// Resident workgroups per CU of that variant (kind = engine_kinds).
 hipError_t relax_occupancy(std::uint32_t block, bool persistent, bool instrumented,
-                           std::uint32_t min_waves, std::uint32_t kind, std::size_t lds_bytes,
-                           int* blocks_per_cu);
-// Per-block-size entry points (relax_b<B>.hip).
-#define BFN_RELAX_SHAPE_DECL(B)                                                               \
-  hipError_t relax_launch_b##B(bool persistent, bool cooperative, std::uint32_t grid,        \
-                               std::size_t lds_bytes, std::uint32_t max_v,                   \
-                               std::uint32_t work_words, const DevGraph& g, const DevBatch& b, \
-                               hipStream_t stream);                                          \
-  hipError_t relax_occupancy_b##B(bool persistent, bool instrumented, std::uint32_t min_waves, \
-                                  std::uint32_t kind, std::size_t lds_bytes, int* blocks_per_cu);
-BFN_RELAX_SHAPE_DECL(64)
-BFN_RELAX_SHAPE_DECL(128)
-BFN_RELAX_SHAPE_DECL(256)
-BFN_RELAX_SHAPE_DECL(512)
-BFN_RELAX_SHAPE_DECL(1024)
-#undef BFN_RELAX_SHAPE_DECL
+                           std::uint32_t min_waves, std::size_t lds_bytes, int* blocks_per_cu);
 hipError_t prep_launch(const DevGraph& g, const DevBatch& b, std::uint64_t pool_words,
                        hipStream_t stream);
 // Inputs and outputs of the tile-image derivation (graph.hip).
@@ -75,7 +57,6 @@ struct TileImageArgs {
   std::uint32_t* csr_perm;
   std::uint32_t* csc_intra;
   std::uint32_t* csr_intra;
-  std::uint32_t* csr_bnd;
   std::uint32_t* stats;
   std::uint32_t* host_in_total;
   std::uint32_t* host_out_total;

@@ -19,7 +19,6 @@
 
 #include <algorithm>
 #include <chrono>
-#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -105,7 +104,7 @@ constexpr std::size_t align_up(std::size_t x, std::size_t a) { return (x + a - 1
 // LDS words of one relaxation workgroup: own + orig labels, the work area
 // (staged slabs, or snapshot + proposals), the table and the flags.
 constexpr std::size_t lds_words(std::uint32_t max_v, std::uint32_t work_words) {
-  return relax_lds_words(max_v, work_words);
+  return 2ull * max_v + work_words + kLdsTableWords + kLdsFlagWords;
 }
 
 bool valid_block(std::uint32_t b) {
@@ -126,7 +125,7 @@ struct bfn_graph_s {
   std::uint32_t V = 0, E = 0, T = 0, Rcsr = 0, Rcsc = 0;
   DeviceBuffer owner, tile_off, row_off, dst, w, row_runs, run_edges, run_dst, col_off, src, cw,
       eid, col_runs, crun_edges, crun_src, in_off, out_off, csc_pack, csr_pack, csc_rec, csr_rec,
-      csc_perm, csr_perm, csc_intra, csr_intra, csr_bnd, base_cost, occ_scratch, counts, stats;
+      csc_perm, csr_perm, csc_intra, csr_intra, base_cost, occ_scratch, counts, stats;
   std::uint32_t* in_tiles = nullptr;
   std::uint32_t* in_stage = nullptr;
   std::uint32_t* out_tiles = nullptr;
@@ -163,7 +162,7 @@ struct bfn_workspace_s {
   // batch metadata in one device buffer + separately grown state
   DeviceBuffer meta, slot_in, slot_out, stamp, lists, pool, last_change, control, status, relax,
       scratch_v, paths_summary, paths_len, paths_off, frames, onpath, clean, arena_v, arena_l,
-      arena_e, occupancy, window_min, slot_desc, slot_out_off;
+      arena_e, occupancy;
   PinnedBuffer staging, h_status, h_control, h_paths;
   // last batch
   bfn_graph graph = nullptr;
@@ -336,7 +335,6 @@ int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_gra
   if (e == hipSuccess) e = g->csr_perm.reserve(sizeof(std::uint32_t) * e_words);
   if (e == hipSuccess) e = g->csc_intra.reserve(sizeof(std::uint32_t) * (T + 1));
   if (e == hipSuccess) e = g->csr_intra.reserve(sizeof(std::uint32_t) * (T + 1));
-  if (e == hipSuccess) e = g->csr_bnd.reserve(sizeof(std::uint32_t) * std::max(V, 1u));
   if (e == hipSuccess) e = g->stats.reserve(sizeof(std::uint32_t) * 8);
   if (e == hipSuccess) e = hipMemsetAsync(g->stats.ptr, 0, sizeof(std::uint32_t) * 8, s);
   if (e != hipSuccess) return cleanup(BFN_E_HIP, std::string("graph upload: ") + hipGetErrorString(e));
@@ -371,7 +369,6 @@ int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_gra
   ta.csr_perm = g->csr_perm.as<std::uint32_t>();
   ta.csc_intra = g->csc_intra.as<std::uint32_t>();
   ta.csr_intra = g->csr_intra.as<std::uint32_t>();
-  ta.csr_bnd = g->csr_bnd.as<std::uint32_t>();
   ta.stats = g->stats.as<std::uint32_t>();
   ta.host_in_total = &in_total;
   ta.host_out_total = &out_total;
@@ -422,7 +419,6 @@ int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_gra
   v.csr_rec = g->csr_rec.as<uint2>();
   v.csc_intra = g->csc_intra.as<std::uint32_t>();
   v.csr_intra = g->csr_intra.as<std::uint32_t>();
-  v.csr_bnd = g->csr_bnd.as<std::uint32_t>();
 
   bfn_graph_info& info = g->info;
   info.max_tile_vertices = max_v;
@@ -437,8 +433,8 @@ int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_gra
   info.max_stage_floats = stats[1];
   info.has_zero_weight = stats[4];
   // reference image + derived tile lists + packs (4 B) + records (8 B) +
-  // record permutations (4 B) per edge and orientation + boundary row starts
-  info.device_bytes = 4ull * (3ull * V + 2 + T + 1 + 6ull * E + 2ull * (g->Rcsr + g->Rcsc + 2) +
+  // record permutations (4 B) per edge and orientation
+  info.device_bytes = 4ull * (2ull * V + 2 + T + 1 + 6ull * E + 2ull * (g->Rcsr + g->Rcsc + 2) +
                               4ull * (T + 1) + 2ull * in_total + 2ull * out_total) +
                       2ull * E * (4 + 8 + 4);
   *out = g;
@@ -709,8 +705,7 @@ std::size_t meta_layout(std::uint32_t Q, std::uint32_t S, std::uint32_t ns, std:
 std::uint64_t batch_bytes(const BatchPlanHost& p, std::uint64_t meta_bytes) {
   return meta_bytes + 4ull * (p.in_total + p.out_total) + 16ull * p.S +
          4ull * p.pool_words + 4ull * p.Q * 2 + sizeof(QueryStatusDev) * p.Q + 8ull * p.Q +
-         sizeof(DevControl) + 8ull * p.S + 12ull * p.Q +  // the label window
-         sizeof(SlotDesc) * p.S + 8ull * p.out_total;    // slot descriptors
+         sizeof(DevControl);
 }
 
 // Sets up the DevBatch view over the workspace buffers.
@@ -726,8 +721,6 @@ void bind_view(bfn_workspace ws, const BatchPlanHost& p, const std::size_t* seg,
   b.instrumentation = o->instrumentation;
   b.boundary_mode = t->boundary_mode;
   b.min_waves = t->min_waves;
-  b.window = p.any_push ? t->window : 0.0f;
-  b.engine_kinds = (p.any_jacobi ? 1u : 0u) | (p.any_push ? 2u : 0u);
   b.tile_begin = reinterpret_cast<const std::uint32_t*>(m + seg[0]);
   b.tile_end = b.tile_begin + 1;
   b.query_tiles = reinterpret_cast<const std::uint32_t*>(m + seg[1]);
@@ -744,15 +737,11 @@ void bind_view(bfn_workspace ws, const BatchPlanHost& p, const std::size_t* seg,
   b.slot_out_base = reinterpret_cast<const std::uint32_t*>(m + seg[10]);
   b.slot_in = ws->slot_in.as<std::uint32_t>();
   b.slot_out = ws->slot_out.as<std::uint32_t>();
-  b.slot_desc = ws->slot_desc.as<SlotDesc>();
-  b.slot_out_off = ws->slot_out_off.as<unsigned long long>();
   b.stamp = ws->stamp.as<std::uint32_t>();
   b.lists = ws->lists.as<std::uint32_t>();
   b.dist = ws->pool.as<std::uint32_t>();
   b.last_change = ws->last_change.as<std::uint32_t>();
   b.pending = nullptr;
-  b.slot_min = b.window > 0.0f ? ws->window_min.as<std::uint32_t>() : nullptr;
-  b.query_min = b.slot_min ? b.slot_min + 2ull * p.S : nullptr;
   b.control = ws->control.as<DevControl>();
 }
 
@@ -899,10 +888,9 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
   if (int32_t rc = validate_options(o)) return rc;
   bfn_tuning t;
   if (tuning) t = *tuning; else bfn_tuning_default(&t);
-  if (t.tile_sweeps > 1 || t.boundary_mode > 1 || t.use_graph > 1 || t.min_waves > 8 ||
-      t.items_per_claim > kMaxItemsPerClaim || !(t.window >= 0.0f) || !std::isfinite(t.window))
-    return fail(BFN_E_INVALID_ARGUMENT, "invalid tuning (tile_sweeps, boundary_mode, use_graph, "
-                                        "min_waves, items_per_claim, window)");
+  if (t.tile_sweeps > 1 || t.boundary_mode > 1 || t.use_graph > 1 || t.min_waves > 8)
+    return fail(BFN_E_INVALID_ARGUMENT,
+                "invalid tuning (tile_sweeps, boundary_mode, use_graph, min_waves)");
   BatchPlanHost p;
   if (int32_t rc = plan_batch(graph, d, o, p)) return rc;
   BFN_HIP(hipSetDevice(ws->device->ordinal));
@@ -938,13 +926,10 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
   BFN_HIP(ws->meta.reserve(meta_bytes));
   BFN_HIP(ws->slot_in.reserve(4ull * std::max<std::uint64_t>(p.in_total, 1)));
   BFN_HIP(ws->slot_out.reserve(4ull * std::max<std::uint64_t>(p.out_total, 1)));
-  BFN_HIP(ws->slot_desc.reserve(sizeof(SlotDesc) * std::max(S, 1u)));
-  BFN_HIP(ws->slot_out_off.reserve(8ull * std::max<std::uint64_t>(p.out_total, 1)));
   BFN_HIP(ws->stamp.reserve(4ull * std::max(S, 1u)));
   BFN_HIP(ws->lists.reserve(12ull * std::max(S, 1u)));
   BFN_HIP(ws->pool.reserve(4ull * std::max<std::uint64_t>(p.pool_words, 1)));
   BFN_HIP(ws->last_change.reserve(4ull * Q));
-  if (p.any_push && t.window > 0.0f) BFN_HIP(ws->window_min.reserve(8ull * S + 12ull * Q));
   BFN_HIP(ws->control.reserve(sizeof(DevControl)));
   BFN_HIP(ws->status.reserve(sizeof(QueryStatusDev) * Q));
   BFN_HIP(ws->relax.reserve(8ull * Q));
@@ -977,10 +962,8 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
                          lds_words(ws->max_v, gv.max_tile_vertices + gv.max_out_stage_floats) * 4ull <=
                              lds_limit;
   if (!proposals) ws->view.boundary_mode = 1;
-  // push, direct boundary: snapshot + per-vertex boundary starts (npt + 1)
   ws->work_words = std::max({p.any_jacobi ? gv.max_stage_floats : 0u, gv.max_tile_vertices,
-                             proposals ? gv.max_tile_vertices + gv.max_out_stage_floats : 0u,
-                             p.any_push && !proposals ? 2 * gv.max_tile_vertices + 1 : 0u});
+                             proposals ? gv.max_tile_vertices + gv.max_out_stage_floats : 0u});
   ws->lds_bytes = 4ull * lds_words(ws->max_v, ws->work_words);
   if (ws->lds_bytes > lds_limit)
     return fail(BFN_E_NOT_REPRESENTABLE, "tile neighbourhood exceeds the LDS budget (" +
@@ -988,7 +971,6 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
                                              std::to_string(lds_limit) + " bytes)");
   int per_cu = 0;
   BFN_HIP(relax_occupancy(ws->block, o->control_mode == 0, o->instrumentation != 0, t.min_waves,
-                          ws->view.engine_kinds,
                           ws->lds_bytes, &per_cu));
   if (per_cu <= 0) return fail(BFN_E_NOT_REPRESENTABLE, "relaxation kernel cannot be resident");
   const std::uint32_t cus = static_cast<std::uint32_t>(ws->device->props.multiProcessorCount);

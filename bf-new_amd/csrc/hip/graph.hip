@@ -214,29 +214,6 @@ __global__ __launch_bounds__(256) void split_records_kernel(SplitArgs a) {
   if (tid == 0) a.intra[T] = n_int;
 }
 
-// Per vertex of each tile, the position of its first boundary record: the
-// boundary part of a tile's block keeps CSR order, so it is the first record
-// whose source is at or after the vertex (binary search on the source).
-__global__ __launch_bounds__(256) void boundary_rows_kernel(const std::uint32_t* tile_vertex_offsets,
-                                                            const std::uint32_t* offsets,
-                                                            const uint2* rec,
-                                                            const std::uint32_t* intra,
-                                                            std::uint32_t* bnd) {
-  const std::uint32_t T = blockIdx.x;
-  const std::uint32_t v0 = tile_vertex_offsets[T];
-  const std::uint32_t npt = tile_vertex_offsets[T + 1] - v0;
-  const std::uint32_t ei = offsets[v0] + intra[T];
-  const std::uint32_t e1 = offsets[v0 + npt];
-  for (std::uint32_t i = threadIdx.x; i < npt; i += blockDim.x) {
-    std::uint32_t lo = ei, hi = e1;
-    while (lo < hi) {
-      const std::uint32_t mid = (lo + hi) >> 1;
-      if (((rec[mid].x >> 12) & kPackLocalMask) < i) lo = mid + 1; else hi = mid;
-    }
-    bnd[v0 + i] = lo;
-  }
-}
-
 __global__ void refresh_weights_kernel(uint2* rec, const std::uint32_t* perm, const float* w,
                                        std::uint32_t n) {
   const std::uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -318,8 +295,6 @@ hipError_t derive_tile_image(const TileImageArgs& t, hipStream_t stream) {
                        t.csr_rec, t.csr_perm, t.csr_intra};
   hipLaunchKernelGGL(split_records_kernel, dim3(t.tile_count), dim3(256), 0, stream, sin);
   hipLaunchKernelGGL(split_records_kernel, dim3(t.tile_count), dim3(256), 0, stream, sout);
-  hipLaunchKernelGGL(boundary_rows_kernel, dim3(t.tile_count), dim3(256), 0, stream,
-                     t.tile_vertex_offsets, t.row_offsets, t.csr_rec, t.csr_intra, t.csr_bnd);
   return hipGetLastError();
 }
 

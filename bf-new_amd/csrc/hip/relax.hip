@@ -1,9 +1,579 @@
-// relax.hip -- batch preparation kernels and the dispatch of the relaxation
-// kernels (relax_kernels.hpp) to their per-block-size translation units.
+// relax.hip -- the three relaxation engines as tile visits, and their control.
+//
+// Work unit: a slot s = (query q, selected tile T).  One workgroup visits one
+// slot at a time.  The tile's edge records (pack, weight) are read once per
+// visit, coalesced: the internal records (both ends in T) into registers,
+// where they stay for the tile closure (thread t holds records t, t+B, ...),
+// the boundary records streamed in chunks.  The distances the visit touches
+// are staged in LDS and relaxed there with ds_min_u32 on the order-preserving
+// u32 image of the fp32 distance (all distances are +0, finite positive or
+// +inf, so unsigned order equals float order).
+//
+// Semantics (reference engines):
+//   Jacobi pull   batched_jacobi.hip.cpp:466-694 -- two columns, round r reads
+//                 column r&1 and writes column (r+1)&1:
+//                 d'[v] = min(d[v], min over admitted in-edges fl(d[u] + w)).
+//                 tile_sweeps = 1 is exactly that (same rounds as the
+//                 reference); tile_sweeps = 0 additionally closes the tile's
+//                 internal edges in LDS each round (block Jacobi: the same
+//                 fixed point, fewer rounds).
+//   dense push    batched_dense_chaotic_push.hip.cpp:392-627 -- one column,
+//                 in place, atomicMin(d[dst], fl(d[src] + w)) over every
+//                 admitted out-edge of a dirty tile.
+//   frontier push batched_frontier_push.hip.cpp:710-906 -- as dense, but
+//                 only from vertices whose label dropped below the label last
+//                 pushed (column 1 holds that snapshot).
+// Every schedule reaches the same bits: fl(x + w) is monotone in x, so the
+// relaxation operator has one greatest fixed point.
+//
+// Admission: a neighbour tile that the query did not select is staged as
+// +inf (pull) or has no slot (push), so edges from or to it never relax.
 #include "bfn_internal.hpp"
-#include "bfn_launch.hpp"
 
 namespace bfn {
+
+// LDS carve-up (u32 words): [own | orig | work | table | flags], with
+// `work` = staged neighbour slabs (pull), or the pushed snapshot followed by
+// the boundary proposal slabs (push), and `table` = per-neighbour LDS offsets
+// (pull) or neighbour slots, "pushed into" marks and proposal slab offsets
+// (push).
+struct Lds {
+  std::uint32_t* own;
+  std::uint32_t* orig;
+  std::uint32_t* work;
+  std::uint32_t* prop;
+  std::uint32_t* table;
+  std::uint32_t* flags;
+};
+
+__device__ __forceinline__ Lds carve(std::uint32_t* base, std::uint32_t max_v,
+                                     std::uint32_t work_words) {
+  Lds l;
+  l.own = base;
+  l.orig = base + max_v;
+  l.work = base + 2 * max_v;
+  l.prop = l.work + max_v;
+  l.table = l.work + work_words;
+  l.flags = l.table + kLdsTableWords;
+  return l;
+}
+
+// Edge records in registers: thread t holds records t, t + B, t + 2B, ... of
+// the range it loaded.
+template <int B, int N>
+struct RecordRegs {
+  std::uint32_t pack[N];
+  float w[N];
+};
+
+// Loads records [base, base + N*B) clipped to `end` (kNone past the end).
+template <int B, int N>
+__device__ __forceinline__ void load_records(RecordRegs<B, N>& r, const uint2* rec,
+                                             std::uint32_t base, std::uint32_t end) {
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    const std::uint32_t e = base + j * B + threadIdx.x;
+    if (e < end) {
+      const uint2 x = rec[e];
+      r.pack[j] = x.x;
+      r.w[j] = __uint_as_float(x.y);
+    } else {
+      r.pack[j] = kNone;
+      r.w[j] = 0.0f;
+    }
+  }
+}
+
+// Boundary records per thread per streamed chunk (loads in flight per lane):
+// more for the large blocks, fewer where registers bound the occupancy.
+constexpr int stream_records(int block) { return block >= 512 ? 8 : 4; }
+
+// Per-workgroup counters of the visits, compiled in for instrumented runs
+// only (I = true; otherwise every counter is dead code).  Phase clocks are
+// shader-clock cycles read by thread 0 at phase boundaries: 0 setup (tables,
+// label staging), 1 internal records (pull: first pass), 2 tile closure,
+// 3 boundary records, 4 write-back and enqueue.
+template <bool I>
+struct VisitStats {
+  unsigned long long attempts = 0, successes = 0, bytes = 0, edges = 0, items = 0, sweeps = 0;
+  unsigned long long phase[kPhaseCount] = {};
+  long long t = 0;
+  __device__ __forceinline__ void start() {
+    if constexpr (I) {
+      if (threadIdx.x == 0) t = clock64();
+    }
+  }
+  __device__ __forceinline__ void mark(int p) {
+    if constexpr (I) {
+      if (threadIdx.x == 0) {
+        const long long now = clock64();
+        phase[p] += static_cast<unsigned long long>(now - t);
+        t = now;
+      }
+    }
+  }
+};
+
+__device__ __forceinline__ std::uint32_t pk_k(std::uint32_t p) { return p >> 24; }
+__device__ __forceinline__ std::uint32_t pk_src(std::uint32_t p) { return (p >> 12) & kPackLocalMask; }
+__device__ __forceinline__ std::uint32_t pk_dst(std::uint32_t p) { return p & kPackLocalMask; }
+
+// ---------------------------------------------------------------------------
+// Jacobi pull visit.
+// ---------------------------------------------------------------------------
+template <int B, int EPT, bool I>
+__device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b, std::uint32_t s,
+                           std::uint32_t r, const Lds& l, RecordRegs<B, EPT>& internal,
+                           std::uint32_t& regs_key, VisitStats<I>& st) {
+  const std::uint32_t tid = threadIdx.x;
+  st.start();
+  const std::uint32_t T = b.tiles[s];
+  const std::uint32_t q = b.slot_query[s];
+  const std::uint32_t v0 = g.tile_vertex_offsets[T];
+  const std::uint32_t npt = g.tile_vertex_offsets[T + 1] - v0;
+  const std::uint32_t nb0 = g.in_nbr_offsets[T];
+  const std::uint32_t K = g.in_nbr_offsets[T + 1] - nb0;
+  const std::uint32_t rp = r & 1u;
+  const std::uint32_t in_base = b.slot_in_base[s];
+
+  for (std::uint32_t k = tid; k < K; k += B) l.table[k] = g.in_nbr_stage[nb0 + k];
+  __syncthreads();
+  // Stage the read column of every in-neighbour slab (k = 0 is T itself).
+  for (std::uint32_t k = 0; k < K; ++k) {
+    const std::uint32_t nt = g.in_nbr_tiles[nb0 + k];
+    const std::uint32_t n = g.tile_vertex_offsets[nt + 1] - g.tile_vertex_offsets[nt];
+    const std::uint32_t ns = b.slot_in[in_base + k];
+    std::uint32_t* dst = l.work + l.table[k];
+    if (ns == kNone) {
+      for (std::uint32_t i = tid; i < n; i += B) dst[i] = kInfBits;
+    } else {
+      if (tid == 0) st.bytes += 4ull * n;
+      const std::uint32_t* src = b.dist + b.slot_dist[ns] + static_cast<std::size_t>(rp) * n;
+      for (std::uint32_t i = tid; i < n; i += B) dst[i] = src[i];
+    }
+  }
+  __syncthreads();
+  for (std::uint32_t i = tid; i < npt; i += B) l.own[i] = l.work[i];
+  __syncthreads();
+  st.mark(0);
+
+  // CSC block of T: internal records [e0, ei), boundary records [ei, e1).
+  const std::uint32_t e0 = g.column_offsets[v0];
+  const std::uint32_t e1 = g.column_offsets[v0 + npt];
+  const std::uint32_t ei = e0 + g.csc_intra[T];
+  const bool resident = ei - e0 <= static_cast<std::uint32_t>(B * EPT);
+  if (tid == 0) {
+    // edge records (pack + weight) once, the written column once
+    st.bytes += 8ull * (e1 - e0) + 4ull * npt;
+    st.edges += e1 - e0;
+  }
+  // Pass 1: every admitted in-edge reads the staged read column.
+  auto relax_staged = [&](std::uint32_t p, float w) {
+    const std::uint32_t cb = as_bits(as_float(l.work[l.table[pk_k(p)] + pk_src(p)]) + w);
+    if (cb < l.own[pk_dst(p)]) atomicMin(&l.own[pk_dst(p)], cb);
+  };
+  // The internal records stay in registers from a previous visit of T.
+  const std::uint32_t key = 2 * T;
+  if (resident && regs_key == key) {
+#pragma unroll
+    for (int j = 0; j < EPT; ++j)
+      if (internal.pack[j] != kNone) relax_staged(internal.pack[j], internal.w[j]);
+  } else {
+    for (std::uint32_t base = e0; base < ei; base += B * EPT) {
+      load_records<B, EPT>(internal, g.csc_rec, base, ei);
+#pragma unroll
+      for (int j = 0; j < EPT; ++j)
+        if (internal.pack[j] != kNone) relax_staged(internal.pack[j], internal.w[j]);
+    }
+    regs_key = resident ? key : kNone;
+  }
+  st.mark(1);
+  for (std::uint32_t base = ei; base < e1; base += B * stream_records(B)) {
+    RecordRegs<B, stream_records(B)> chunk;
+    load_records<B, stream_records(B)>(chunk, g.csc_rec, base, e1);
+#pragma unroll
+    for (int j = 0; j < stream_records(B); ++j)
+      if (chunk.pack[j] != kNone) relax_staged(chunk.pack[j], chunk.w[j]);
+  }
+  __syncthreads();
+  st.mark(3);
+  if (b.tile_sweeps == 0) {
+    // Close the tile's internal edges in place; the internal records stay in
+    // registers when they fit.
+    for (;;) {
+      bool changed = false;
+      for (std::uint32_t base = e0; base < ei; base += B * EPT) {
+        if (!resident) load_records<B, EPT>(internal, g.csc_rec, base, ei);
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) {
+          const std::uint32_t p = internal.pack[j];
+          if (p == kNone) continue;
+          const std::uint32_t cb = as_bits(as_float(l.own[pk_src(p)]) + internal.w[j]);
+          if (cb < l.own[pk_dst(p)]) {
+            atomicMin(&l.own[pk_dst(p)], cb);
+            changed = true;
+          }
+        }
+      }
+      ++st.sweeps;
+      if (!__syncthreads_or(changed)) break;
+    }
+  }
+  st.mark(2);
+
+  std::uint32_t* out = b.dist + b.slot_dist[s] + static_cast<std::size_t>(rp ^ 1u) * npt;
+  bool changed = false;
+  for (std::uint32_t i = tid; i < npt; i += B) {
+    const std::uint32_t v = l.own[i];
+    out[i] = v;
+    changed |= (v != l.work[i]);
+  }
+  if (__syncthreads_or(changed)) {
+    if (tid == 0) atomic_max_agent(&b.last_change[q], r + 1);
+    const std::uint32_t ob0 = g.out_nbr_offsets[T];
+    const std::uint32_t KO = g.out_nbr_offsets[T + 1] - ob0;
+    const std::uint32_t out_base = b.slot_out_base[s];
+    for (std::uint32_t j = tid; j < KO; j += B) {
+      const std::uint32_t ns = b.slot_out[out_base + j];
+      if (ns != kNone) enqueue(b, ns, r + 1);
+    }
+  }
+  st.mark(4);
+}
+
+// ---------------------------------------------------------------------------
+// Dense / frontier push visit.
+// ---------------------------------------------------------------------------
+template <int B, int EPT, bool I>
+__device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b, std::uint32_t s,
+                           std::uint32_t r, bool frontier, const Lds& l,
+                           RecordRegs<B, EPT>& internal, std::uint32_t& regs_key,
+                           VisitStats<I>& st) {
+  const std::uint32_t tid = threadIdx.x;
+  st.start();
+  const std::uint32_t T = b.tiles[s];
+  const std::uint32_t q = b.slot_query[s];
+  const std::uint32_t v0 = g.tile_vertex_offsets[T];
+  const std::uint32_t npt = g.tile_vertex_offsets[T + 1] - v0;
+  const std::uint32_t ob0 = g.out_nbr_offsets[T];
+  const std::uint32_t KO = g.out_nbr_offsets[T + 1] - ob0;
+  const std::uint32_t out_base = b.slot_out_base[s];
+  std::uint32_t* column = b.dist + b.slot_dist[s];
+  std::uint32_t* snapshot = column + npt;
+
+  const bool proposals = b.boundary_mode == 0;
+  for (std::uint32_t k = tid; k < KO; k += B) {
+    const std::uint32_t ns = b.slot_out[out_base + k];
+    l.table[k] = ns;
+    l.table[256 + k] = 0;
+    if (proposals) {
+      l.table[512 + k] = g.out_nbr_stage[ob0 + k];
+    } else if (ns != kNone) {
+      // pool offset of the neighbour's labels, as two words
+      const unsigned long long off = b.slot_dist[ns];
+      l.table[512 + 2 * k] = static_cast<std::uint32_t>(off);
+      l.table[513 + 2 * k] = static_cast<std::uint32_t>(off >> 32);
+    }
+  }
+  for (std::uint32_t i = tid; i < npt; i += B) {
+    const std::uint32_t x = load_agent(column + i);
+    l.own[i] = x;
+    l.orig[i] = x;
+    // Dense pushes every finite vertex: an infinite snapshot admits them all.
+    l.work[i] = frontier ? snapshot[i] : kInfBits;
+  }
+  if (proposals) {
+    // Clear the proposal slab of every selected out-neighbour (k >= 1).
+    for (std::uint32_t k = 1; k < KO; ++k) {
+      if (b.slot_out[out_base + k] == kNone) continue;
+      const std::uint32_t nt = g.out_nbr_tiles[ob0 + k];
+      const std::uint32_t n = g.tile_vertex_offsets[nt + 1] - g.tile_vertex_offsets[nt];
+      std::uint32_t* slab = l.prop + g.out_nbr_stage[ob0 + k];
+      for (std::uint32_t i = tid; i < n; i += B) slab[i] = kInfBits;
+    }
+  }
+  __syncthreads();
+  st.mark(0);
+
+  // CSR block of T: internal records [e0, ei), boundary records [ei, e1).
+  const std::uint32_t e0 = g.row_offsets[v0];
+  const std::uint32_t e1 = g.row_offsets[v0 + npt];
+  const std::uint32_t ei = e0 + g.csr_intra[T];
+  const bool resident = ei - e0 <= static_cast<std::uint32_t>(B * EPT);
+  if (tid == 0) {
+    // edge records once, the column (and the snapshot for frontier) once
+    st.bytes += 8ull * (e1 - e0) + 4ull * npt * (frontier ? 2 : 1);
+    st.edges += e1 - e0;
+  }
+  // The internal records stay in registers from a previous visit of T.
+  const std::uint32_t key = 2 * T + 1;
+  if (resident && regs_key != key) load_records<B, EPT>(internal, g.csr_rec, e0, ei);
+  regs_key = resident ? key : kNone;
+  st.mark(1);
+  // Internal edges to the tile's local fixed point, from frontier sources.
+  for (;;) {
+    bool changed = false;
+    for (std::uint32_t base = e0; base < ei; base += B * EPT) {
+      if (!resident) load_records<B, EPT>(internal, g.csr_rec, base, ei);
+#pragma unroll
+      for (int j = 0; j < EPT; ++j) {
+        const std::uint32_t p = internal.pack[j];
+        if (p == kNone) continue;
+        const std::uint32_t sv = l.own[pk_src(p)];
+        if (!(sv < l.work[pk_src(p)])) continue;
+        const std::uint32_t cb = as_bits(as_float(sv) + internal.w[j]);
+        if (cb < l.own[pk_dst(p)]) {
+          atomicMin(&l.own[pk_dst(p)], cb);
+          changed = true;
+        }
+      }
+    }
+    ++st.sweeps;
+    if (!__syncthreads_or(changed)) break;
+  }
+  st.mark(2);
+  // Boundary records, streamed once.  Labels only fall during a run, so an
+  // ordinary (cacheable, possibly stale) load of a neighbour label is never
+  // below the current label: comparing against it never drops a decrease, at
+  // worst it issues an atomic or marks a neighbour that did not need it.
+  unsigned long long att = 0, suc = 0;
+  if (proposals) {
+    // Gather the best candidate per destination in LDS, then compare each
+    // neighbour slab once, coalesced, and issue one atomic per improvement.
+    for (std::uint32_t base = ei; base < e1; base += B * stream_records(B)) {
+      RecordRegs<B, stream_records(B)> chunk;
+      load_records<B, stream_records(B)>(chunk, g.csr_rec, base, e1);
+#pragma unroll
+      for (int j = 0; j < stream_records(B); ++j) {
+        const std::uint32_t p = chunk.pack[j];
+        if (p == kNone) continue;
+        const std::uint32_t sv = l.own[pk_src(p)];
+        if (!(sv < l.work[pk_src(p)])) continue;
+        if (l.table[pk_k(p)] == kNone) continue;
+        const std::uint32_t cb = as_bits(as_float(sv) + chunk.w[j]);
+        std::uint32_t* slot = &l.prop[l.table[512 + pk_k(p)] + pk_dst(p)];
+        ++att;
+        if (cb < *slot) atomicMin(slot, cb);
+      }
+    }
+    __syncthreads();
+    for (std::uint32_t k = 1; k < KO; ++k) {
+      const std::uint32_t ns = l.table[k];
+      if (ns == kNone) continue;
+      const std::uint32_t nt = g.out_nbr_tiles[ob0 + k];
+      const std::uint32_t n = g.tile_vertex_offsets[nt + 1] - g.tile_vertex_offsets[nt];
+      const std::uint32_t* slab = l.prop + l.table[512 + k];
+      std::uint32_t* target = b.dist + b.slot_dist[ns];
+      if (tid == 0) st.bytes += 4ull * n;  // the neighbour slab, read once
+      bool hit = false;
+      for (std::uint32_t i = tid; i < n; i += B) {
+        const std::uint32_t pv = slab[i];
+        if (pv != kInfBits && pv < target[i]) {
+          atomic_min_agent(target + i, pv);
+          ++suc;
+          hit = true;
+        }
+      }
+      if (hit) l.table[256 + k] = 1;
+    }
+  } else {
+    // One global atomic per boundary candidate that beats the neighbour's
+    // label as read.  Per chunk the reads are issued together, then the
+    // atomics, which return nothing: a neighbour is marked when the read
+    // label was above the candidate, which covers every actual decrease.
+    auto candidate = [&](std::uint32_t p, float w, std::uint32_t*& target) -> std::uint32_t {
+      if (p == kNone) return kNone;
+      const std::uint32_t sv = l.own[pk_src(p)];
+      if (!(sv < l.work[pk_src(p)]) || l.table[pk_k(p)] == kNone) return kNone;
+      const unsigned long long off = (static_cast<unsigned long long>(l.table[513 + 2 * pk_k(p)]) << 32) |
+                                     l.table[512 + 2 * pk_k(p)];
+      target = b.dist + off + pk_dst(p);
+      return as_bits(as_float(sv) + w);
+    };
+    for (std::uint32_t base = ei; base < e1; base += B * stream_records(B)) {
+      RecordRegs<B, stream_records(B)> chunk;
+      load_records<B, stream_records(B)>(chunk, g.csr_rec, base, e1);
+      std::uint32_t seen[stream_records(B)];
+#pragma unroll
+      for (int j = 0; j < stream_records(B); ++j) {
+        std::uint32_t* target = nullptr;
+        seen[j] = candidate(chunk.pack[j], chunk.w[j], target) == kNone ? 0u : *target;
+      }
+#pragma unroll
+      for (int j = 0; j < stream_records(B); ++j) {
+        std::uint32_t* target = nullptr;
+        const std::uint32_t cb = candidate(chunk.pack[j], chunk.w[j], target);
+        if (cb == kNone) continue;
+        ++att;
+        if (cb < seen[j]) {
+          atomic_min_agent(target, cb);
+          ++suc;
+          l.table[256 + pk_k(chunk.pack[j])] = 1;
+        }
+      }
+    }
+  }
+  st.attempts += att;
+  st.successes += suc;
+  // direct: a remote label read per attempt; both: read-modify-write per
+  // atomic issued (proposal slabs read once, counted above)
+  st.bytes += (proposals ? 0ull : 4ull * att) + 8ull * suc;
+  st.mark(3);
+  bool changed = false;
+  for (std::uint32_t i = tid; i < npt; i += B) {
+    const std::uint32_t v = l.own[i];
+    if (v < l.orig[i]) {
+      atomic_min_agent(column + i, v);
+      changed = true;
+    }
+    if (frontier && v < l.work[i]) snapshot[i] = v;
+  }
+  const bool pushed_any = __syncthreads_or(changed);
+  bool marked = false;
+  for (std::uint32_t k = 1 + tid; k < KO; k += B) {
+    if (l.table[256 + k]) {
+      enqueue(b, l.table[k], r + 1);
+      marked = true;
+    }
+  }
+  if (__syncthreads_or(marked) || pushed_any) {
+    if (tid == 0) atomic_max_agent(&b.last_change[q], r + 1);
+  }
+  st.mark(4);
+}
+
+// ---------------------------------------------------------------------------
+// Round processing shared by the persistent and host-polled kernels.
+// ---------------------------------------------------------------------------
+template <int B, int EPT, bool I>
+__device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch& b, std::uint32_t r,
+                              std::uint32_t n, const Lds& l) {
+  const std::uint32_t li = r % 3;
+  const std::uint32_t chunk = b.items_per_claim ? b.items_per_claim : 1u;
+  RecordRegs<B, EPT> regs;
+  std::uint32_t regs_key = kNone;  // tile and orientation of the records in regs
+  VisitStats<I> st;
+  for (;;) {
+    if (threadIdx.x == 0) l.flags[0] = atomic_add_agent(&b.control->claim[li], chunk);
+    __syncthreads();
+    const std::uint32_t start = l.flags[0];
+    __syncthreads();
+    if (start >= n) break;
+    const std::uint32_t end = min(start + chunk, n);
+    for (std::uint32_t i = start; i < end; ++i) {
+      const std::uint32_t s = b.lists[static_cast<std::size_t>(li) * b.slot_count + i];
+      const std::uint32_t engine = b.engine[b.slot_query[s]];
+      if (engine == kJacobiPull) {
+        visit_pull<B, EPT, I>(g, b, s, r, l, regs, regs_key, st);
+      } else {
+        visit_push<B, EPT, I>(g, b, s, r, engine == kFrontierPush, l, regs, regs_key, st);
+      }
+      ++st.items;
+      __syncthreads();
+      // the visit's own enqueues are issued before its item is retired
+      if (b.pending && threadIdx.x == 0) atomic_add_agent(&b.pending[b.slot_query[s]], 0xFFFFFFFFu);
+    }
+  }
+  if constexpr (I) {
+    // One add per thread keeps the counters off the hot path's critical lanes.
+    DevControl* c = b.control;
+    if (st.attempts) atomicAdd(&c->atomic_attempts, st.attempts);
+    if (st.successes) atomicAdd(&c->atomic_successes, st.successes);
+    if (threadIdx.x == 0) {
+      if (st.items) atomicAdd(&c->items_processed, st.items);
+      if (st.bytes) atomicAdd(&c->algorithmic_bytes, st.bytes);
+      if (st.edges) atomicAdd(&c->edges_relaxed, st.edges);
+      if (st.sweeps) atomicAdd(&c->closure_sweeps, st.sweeps);
+      for (int p = 0; p < kPhaseCount; ++p)
+        if (st.phase[p]) atomicAdd(&c->phase_cycles[p], st.phase[p]);
+    }
+  }
+}
+
+__device__ __forceinline__ void grid_barrier(DevControl* c, std::uint32_t blocks) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const std::uint32_t gen = load_agent(&c->barrier_generation);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const std::uint32_t arrived = atomic_add_agent(&c->barrier_count, 1u);
+    if (arrived == blocks - 1) {
+      store_agent(&c->barrier_count, 0u);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      store_agent(&c->barrier_generation, gen + 1);
+    } else {
+      while (load_agent(&c->barrier_generation) == gen) __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+// One cooperative launch runs every round: process list r%3, grid barrier,
+// repeat while the next list is non-empty and the round limit allows.
+template <int B, int EPT, bool I, int MW>
+__global__ __launch_bounds__(B, MW) void relax_persistent_kernel(DevGraph g, DevBatch b,
+                                                             std::uint32_t max_v,
+                                                             std::uint32_t work_words) {
+  extern __shared__ std::uint32_t lds_words[];
+  const Lds l = carve(lds_words, max_v, work_words);
+  DevControl* c = b.control;
+  std::uint32_t r = load_agent(&c->round);
+  if (load_agent(&c->done)) return;
+  std::uint32_t n = 0;
+  for (;;) {
+    n = load_agent(&c->list_count[r % 3]);
+    if (n == 0 || (b.maximum_rounds && r >= b.maximum_rounds)) break;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      store_agent(&c->list_count[(r + 2) % 3], 0u);
+      store_agent(&c->claim[(r + 2) % 3], 0u);
+      if (b.instrumentation) atomicMax(&c->max_list, static_cast<unsigned long long>(n));
+    }
+    process_round<B, EPT, I>(g, b, r, n, l);
+    grid_barrier(c, gridDim.x);
+    ++r;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    c->round = r;
+    c->rounds_executed = r;
+    c->done = 1;
+    c->stop_reason = n == 0 ? kStopConverged : kStopMaximumRounds;
+  }
+}
+
+// Host-polled control: one round per launch (ordinary grid), then the
+// one-thread advance (jacobi_pull.hpp:153-220 shape: count the round, reset
+// the list that the next round fills, stop on an empty list or the limit).
+template <int B, int EPT, bool I, int MW>
+__global__ __launch_bounds__(B, MW) void relax_round_kernel(DevGraph g, DevBatch b,
+                                                        std::uint32_t max_v,
+                                                        std::uint32_t work_words) {
+  extern __shared__ std::uint32_t lds_words[];
+  const Lds l = carve(lds_words, max_v, work_words);
+  DevControl* c = b.control;
+  if (load_agent(&c->done)) return;
+  const std::uint32_t r = load_agent(&c->round);
+  const std::uint32_t n = load_agent(&c->list_count[r % 3]);
+  process_round<B, EPT, I>(g, b, r, n, l);
+}
+
+__global__ void relax_advance_kernel(DevBatch b) {
+  DevControl* c = b.control;
+  if (c->done) return;
+  const std::uint32_t r = c->round;
+  const std::uint32_t n = c->list_count[r % 3];
+  if (b.instrumentation && n > c->max_list) c->max_list = n;
+  c->list_count[(r + 2) % 3] = 0;
+  c->claim[(r + 2) % 3] = 0;
+  c->round = r + 1;
+  c->rounds_executed = r + 1;
+  if (c->list_count[(r + 1) % 3] == 0) {
+    c->done = 1;
+    c->stop_reason = kStopConverged;
+  } else if (b.maximum_rounds && r + 1 >= b.maximum_rounds) {
+    c->done = 1;
+    c->stop_reason = kStopMaximumRounds;
+  }
+}
 
 // ---------------------------------------------------------------------------
 // Batch preparation.
@@ -53,7 +623,6 @@ __global__ void prep_slots_kernel(DevGraph g, DevBatch b) {
   const std::uint32_t o0 = g.out_nbr_offsets[T], o1 = g.out_nbr_offsets[T + 1];
   const std::uint32_t obs = b.slot_out_base[s];
   for (std::uint32_t k = o0; k < o1; ++k) b.slot_out[obs + (k - o0)] = find_slot(b, q, g.out_nbr_tiles[k]);
-  fill_slot_desc(g, b, s);
   b.stamp[s] = 0;
   b.last_change[q] = 0;
 }
@@ -77,7 +646,7 @@ __global__ void seed_kernel(DevGraph g, DevBatch b) {
     std::uint32_t* column = b.dist + b.slot_dist[s];
     column[u - v0] = 0u;
     if (jacobi) column[npt + (u - v0)] = 0u;
-    if (jacobi) enqueue(b, s, 0); else enqueue_pushed(b, s, 0, 0u);
+    enqueue(b, s, 0);
     if (jacobi) {
       const std::uint32_t ob = b.slot_out_base[s];
       const std::uint32_t KO = g.out_nbr_offsets[T + 1] - g.out_nbr_offsets[T];
@@ -89,29 +658,81 @@ __global__ void seed_kernel(DevGraph g, DevBatch b) {
   }
 }
 
-hipError_t relax_launch(std::uint32_t block, bool persistent, bool cooperative,
-                        std::uint32_t grid, std::size_t lds_bytes, std::uint32_t max_v,
-                        std::uint32_t work_words, const DevGraph& g, const DevBatch& b,
-                        hipStream_t stream) {
+// ---------------------------------------------------------------------------
+// Launch wrappers (host side of this translation unit).
+// ---------------------------------------------------------------------------
+template <int B, int EPT, bool I, int MW>
+hipError_t launch_relax(bool persistent, bool cooperative, std::uint32_t grid,
+                        std::size_t lds_bytes, std::uint32_t max_v, std::uint32_t work_words,
+                        const DevGraph& g, const DevBatch& b, hipStream_t stream) {
+  if (persistent) {
+    void* args[] = {const_cast<DevGraph*>(&g), const_cast<DevBatch*>(&b), &max_v, &work_words};
+    const void* kernel = reinterpret_cast<const void*>(&relax_persistent_kernel<B, EPT, I, MW>);
+    if (cooperative)
+      return hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(B), args,
+                                        static_cast<unsigned>(lds_bytes), stream);
+    return hipLaunchKernel(kernel, dim3(grid), dim3(B), args, lds_bytes, stream);
+  }
+  hipLaunchKernelGGL((relax_round_kernel<B, EPT, I, MW>), dim3(grid), dim3(B), lds_bytes, stream,
+                     g, b, max_v, work_words);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(relax_advance_kernel, dim3(1), dim3(1), 0, stream, b);
+  return hipGetLastError();
+}
+
+template <int B, int EPT, bool I, int MW>
+hipError_t occupancy_relax(bool persistent, std::size_t lds_bytes, int* blocks_per_cu) {
+  const void* kernel =
+      persistent ? reinterpret_cast<const void*>(&relax_persistent_kernel<B, EPT, I, MW>)
+                 : reinterpret_cast<const void*>(&relax_round_kernel<B, EPT, I, MW>);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, kernel, B, lds_bytes);
+}
+
+// Instantiated kernel variants.  Block sizes come with the registers per
+// thread that hold a tile's internal records (B * EPT = 2048: 512-vertex
+// tiles with four internal out-edges per vertex stay register-resident).
+// The 256- and 512-thread blocks also come bounded to at least 6 or 8 waves
+// per SIMD (fewer registers per lane, more visits in flight); instrumented
+// runs use the unbounded variant.
+template <int B, int EPT, typename F>
+hipError_t with_waves(bool instrumented, std::uint32_t min_waves, F& f) {
+  if (instrumented) return f.template operator()<B, EPT, true, 1>();
+  if constexpr (B == 256 || B == 512) {
+    if (min_waves >= 8) return f.template operator()<B, EPT, false, 8>();
+    if (min_waves >= 6) return f.template operator()<B, EPT, false, 6>();
+  }
+  return f.template operator()<B, EPT, false, 1>();
+}
+
+template <typename F>
+hipError_t with_shape(std::uint32_t block, bool instrumented, std::uint32_t min_waves, F&& f) {
   switch (block) {
-#define BFN_CASE(B) \
-  case B: return relax_launch_b##B(persistent, cooperative, grid, lds_bytes, max_v, work_words, g, b, stream);
-    BFN_CASE(64) BFN_CASE(128) BFN_CASE(256) BFN_CASE(512) BFN_CASE(1024)
-#undef BFN_CASE
+    case 64: return with_waves<64, 32>(instrumented, min_waves, f);
+    case 128: return with_waves<128, 16>(instrumented, min_waves, f);
+    case 256: return with_waves<256, 8>(instrumented, min_waves, f);
+    case 512: return with_waves<512, 4>(instrumented, min_waves, f);
+    case 1024: return with_waves<1024, 2>(instrumented, min_waves, f);
     default: return hipErrorInvalidValue;
   }
 }
 
+hipError_t relax_launch(std::uint32_t block, bool persistent, bool cooperative,
+                        std::uint32_t grid, std::size_t lds_bytes, std::uint32_t max_v,
+                        std::uint32_t work_words, const DevGraph& g, const DevBatch& b,
+                        hipStream_t stream) {
+  return with_shape(block, b.instrumentation != 0, b.min_waves,
+                    [&]<int B, int EPT, bool I, int MW>() {
+                      return launch_relax<B, EPT, I, MW>(persistent, cooperative, grid, lds_bytes,
+                                                         max_v, work_words, g, b, stream);
+                    });
+}
+
 hipError_t relax_occupancy(std::uint32_t block, bool persistent, bool instrumented,
-                           std::uint32_t min_waves, std::uint32_t kind, std::size_t lds_bytes,
-                           int* blocks_per_cu) {
-  switch (block) {
-#define BFN_CASE(B) \
-  case B: return relax_occupancy_b##B(persistent, instrumented, min_waves, kind, lds_bytes, blocks_per_cu);
-    BFN_CASE(64) BFN_CASE(128) BFN_CASE(256) BFN_CASE(512) BFN_CASE(1024)
-#undef BFN_CASE
-    default: return hipErrorInvalidValue;
-  }
+                           std::uint32_t min_waves, std::size_t lds_bytes, int* blocks_per_cu) {
+  return with_shape(block, instrumented, min_waves, [&]<int B, int EPT, bool I, int MW>() {
+    return occupancy_relax<B, EPT, I, MW>(persistent, lds_bytes, blocks_per_cu);
+  });
 }
 
 hipError_t prep_launch(const DevGraph& g, const DevBatch& b, std::uint64_t pool_words,
@@ -123,9 +744,6 @@ hipError_t prep_launch(const DevGraph& g, const DevBatch& b, std::uint64_t pool_
   if (b.slot_count)
     hipLaunchKernelGGL(prep_slots_kernel, dim3((b.slot_count + 255) / 256), dim3(256), 0,
                        stream, g, b);
-  if (b.slot_min)
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(256), dim3(256), 0, stream, b.slot_min,
-                       2ull * b.slot_count + 3ull * b.query_count, kInfBits);
   if (b.query_count)
     hipLaunchKernelGGL(seed_kernel, dim3((b.query_count + 127) / 128), dim3(128), 0, stream, g,
                        b);

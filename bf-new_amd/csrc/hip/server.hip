@@ -83,7 +83,6 @@ __global__ void admit_kernel(DevGraph g, DevBatch b, const AdmitRecord* records,
     const std::uint32_t o0 = g.out_nbr_offsets[T], o1 = g.out_nbr_offsets[T + 1];
     for (std::uint32_t k = o0; k < o1; ++k)
       b.slot_out[static_cast<std::size_t>(s) * kout + (k - o0)] = find_slot(b, q, g.out_nbr_tiles[k]);
-    fill_slot_desc(g, b, s);
   }
   const unsigned long long w0 = static_cast<unsigned long long>(qb) * slab_words;
   const unsigned long long w1 = static_cast<unsigned long long>(qe) * slab_words;
@@ -224,9 +223,7 @@ struct bfn_server_s {
                 *d_last = nullptr, *d_pending = nullptr, *d_active = nullptr, *d_done = nullptr,
                 *d_result = nullptr, *d_clean = nullptr, *d_summary = nullptr,
                 *d_upload_tiles = nullptr;
-  unsigned long long *d_slot_dist = nullptr, *d_frame_base = nullptr, *d_lengths = nullptr,
-                     *d_out_off = nullptr;
-  SlotDesc* d_desc = nullptr;
+  unsigned long long *d_slot_dist = nullptr, *d_frame_base = nullptr, *d_lengths = nullptr;
   DevControl* d_control = nullptr;
   AdmitRecord* d_records = nullptr;
   // pinned staging
@@ -441,10 +438,8 @@ int32_t bfn_server_create(bfn_device device, bfn_graph graph, const bfn_server_o
   sv->slab_words = 2 * sv->max_v;
   // Push engines push boundary edges with direct atomics here (no proposal
   // area), which keeps more workgroups resident for small requests.
-  // push: snapshot + per-vertex boundary starts (npt + 1)
-  sv->work_words = o->engine == kJacobiPull ? std::max(sv->gview.max_stage_floats, sv->max_v)
-                                            : 2 * sv->max_v + 1;
-  sv->lds_bytes = 4ull * relax_lds_words(sv->max_v, sv->work_words);
+  sv->work_words = std::max(o->engine == kJacobiPull ? sv->gview.max_stage_floats : 0u, sv->max_v);
+  sv->lds_bytes = 4ull * (2ull * sv->max_v + sv->work_words + kLdsTableWords + kLdsFlagWords);
   if (sv->lds_bytes > device_lds_limit(device))
     return bail(BFN_E_NOT_REPRESENTABLE, "tile neighbourhood exceeds the LDS budget");
   sv->block = o->block_size;
@@ -476,7 +471,6 @@ int32_t bfn_server_create(bfn_device device, bfn_graph graph, const bfn_server_o
             alloc(&sv->d_stamp, 4 * S) && alloc(&sv->d_lists, 12 * S) &&
             alloc(&sv->d_pool, 4 * S * sv->slab_words) && alloc(&sv->d_onpath, 4 * S * sv->slab_words) &&
             alloc(&sv->d_frames, 12 * S * sv->max_v) && alloc(&sv->d_slot_dist, 8 * S) &&
-            alloc(&sv->d_desc, sizeof(SlotDesc) * S) && alloc(&sv->d_out_off, 8 * S * sv->kout) &&
             alloc(&sv->d_begin, 4 * Q) && alloc(&sv->d_end, 4 * Q) && alloc(&sv->d_src_off, 4 * (Q + 1)) &&
             alloc(&sv->d_src, 4 * Q) && alloc(&sv->d_tgt_off, 4 * (Q + 1)) && alloc(&sv->d_tgt, 4 * Q) &&
             alloc(&sv->d_engine, 4 * Q) && alloc(&sv->d_last, 4 * Q) && alloc(&sv->d_pending, 4 * Q) &&
@@ -502,7 +496,6 @@ int32_t bfn_server_create(bfn_device device, bfn_graph graph, const bfn_server_o
   b.instrumentation = 0;
   b.boundary_mode = 1;
   b.min_waves = 0;
-  b.engine_kinds = o->engine == kJacobiPull ? 1u : 2u;  // one engine per server
   // A request's slots form one contiguous range of the arena, in tile
   // order, so positions and slots coincide.
   b.tile_begin = sv->d_begin;
@@ -517,8 +510,6 @@ int32_t bfn_server_create(bfn_device device, bfn_graph graph, const bfn_server_o
   b.engine = sv->d_engine;
   b.slot_query = sv->d_slot_query;
   b.slot_dist = sv->d_slot_dist;
-  b.slot_desc = sv->d_desc;
-  b.slot_out_off = sv->d_out_off;
   b.slot_in_base = sv->d_in_base;
   b.slot_out_base = sv->d_out_base;
   b.slot_in = sv->d_slot_in;
@@ -534,8 +525,7 @@ int32_t bfn_server_create(bfn_device device, bfn_graph graph, const bfn_server_o
                      sv->Scap, sv->slab_words, sv->kin, sv->kout, sv->d_active, sv->d_src_off,
                      sv->d_tgt_off, sv->d_clean);
   int per_cu = 0;
-  if (relax_occupancy(sv->block, false, false, 0, o->engine == kJacobiPull ? 1u : 2u,
-                      sv->lds_bytes, &per_cu) != hipSuccess ||
+  if (relax_occupancy(sv->block, false, false, 0, sv->lds_bytes, &per_cu) != hipSuccess ||
       per_cu <= 0)
     return bail(BFN_E_NOT_REPRESENTABLE, "relaxation kernel cannot be resident");
   sv->grid = static_cast<std::uint32_t>(per_cu) * device_cu_count(device);

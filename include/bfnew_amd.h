@@ -126,8 +126,7 @@ typedef struct {
   uint32_t tile_sweeps;          /* Jacobi: 1 = point Jacobi (reference round
                                     semantics), 0 = sweep each tile to its local
                                     fixed point every round (block Jacobi) */
-  uint32_t items_per_claim;      /* work items taken per queue claim (0-8;
-                                    0 means 1) */
+  uint32_t items_per_claim;      /* work items taken per queue claim */
   uint32_t graph_rounds;         /* rounds captured per hipGraph replay */
   uint32_t use_graph;            /* replay the batch step through a hipGraph */
   uint32_t count_relaxations;    /* compute per-query relaxation counts */
@@ -139,11 +138,7 @@ typedef struct {
   uint32_t min_waves;            /* 0 = compiler default; 6 or 8 = the 256- and
                                     512-thread kernels compiled for at least that
                                     many waves per SIMD */
-  float window;                  /* push engines: visit a queued tile only when
-                                    the smallest label pushed into it is below
-                                    its query's smallest queued label plus this
-                                    window (0 = every queued tile each round);
-                                    any window gives the same distances */
+  uint32_t reserved;
 } bfn_tuning;
 
 /* The reference DeviceGraphLayout32 (device_layout.hpp:72-93) plus the tile

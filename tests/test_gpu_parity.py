@@ -18,8 +18,7 @@ CONTROLS = (bfn.PERSISTENT, bfn.CHUNKED, bfn.PER_ROUND)
 
 def configurations():
     """(engine, control, knobs): Jacobi in point and block mode, the push
-    engines with LDS boundary proposals and with direct atomics, each also
-    with a narrow label window (tiles deferred to later rounds)."""
+    engines with LDS boundary proposals and with direct atomics."""
     for eng, ctl in itertools.product(ENGINES, CONTROLS):
         if eng == bfn.JACOBI_PULL:
             for sweeps in (1, 0):
@@ -27,7 +26,6 @@ def configurations():
         else:
             for mode in (0, 1):
                 yield eng, ctl, {"boundary_mode": mode}
-                yield eng, ctl, {"boundary_mode": mode, "window": 0.25}
 
 
 def fixture_graph(device, case):
@@ -211,11 +209,7 @@ def test_c2_batched_and_plain_bit_exact(device):
     for eng, knobs in ((0, {"sweeps": 0}), (0, {"sweeps": 1}), (1, {"boundary_mode": 0}),
                        (1, {"boundary_mode": 1}), (2, {"boundary_mode": 0}),
                        (2, {"boundary_mode": 1}), (2, {"boundary_mode": 1, "min_waves": 6}),
-                       (1, {"boundary_mode": 1, "min_waves": 8}),
-                       (2, {"boundary_mode": 1, "window": 0.5}),
-                       (2, {"boundary_mode": 1, "window": 4.0}),
-                       (2, {"boundary_mode": 0, "window": 2.0}),
-                       (1, {"boundary_mode": 1, "window": 1.0})):
+                       (1, {"boundary_mode": 1, "min_waves": 8})):
         ws = run(device, dg, batch, eng, bfn.PERSISTENT, **knobs)
         for i in range(len(qs)):
             assert np.array_equal(bits(ws.distances(i)), bits(refs[i])), (eng, knobs, i)
@@ -239,11 +233,6 @@ def test_mixed_engines_and_rerun(device):
     first = [ws.distances(i) for i in range(len(qs))]
     for i in range(len(qs)):
         assert np.array_equal(bits(first[i]), bits(refs[i])), i
-    # the label window applies to the push queries only
-    windowed = run(device, dg, batch, bfn.JACOBI_PULL, bfn.CHUNKED, 0, window=0.5)
-    for i in range(len(qs)):
-        assert np.array_equal(bits(windowed.distances(i)), bits(refs[i])), i
-    windowed.close()
     for _ in range(3):
         ws.rerun()
         for i in range(len(qs)):
@@ -257,12 +246,11 @@ def test_hipgraph_replay(device):
     view = hg.view()
     qs, batch = c2_batch(hg, 8, seed=7)
     refs = [oracle_query(view, [q.source], batch.tiles[i])[0] for i, q in enumerate(qs)]
-    for eng, window in [(e, 0.0) for e in ENGINES] + [(bfn.FRONTIER_PUSH, 1.0)]:
-        ws = run(device, dg, batch, eng, bfn.CHUNKED, 0, use_graph=1, graph_rounds=3,
-                 window=window)
+    for eng in ENGINES:
+        ws = run(device, dg, batch, eng, bfn.CHUNKED, 0, use_graph=1, graph_rounds=3)
         for rep in range(3):
             for i in range(len(qs)):
-                assert np.array_equal(bits(ws.distances(i)), bits(refs[i])), (eng, window, rep, i)
+                assert np.array_equal(bits(ws.distances(i)), bits(refs[i])), (eng, rep, i)
             ws.rerun()
         ws.close()
 
@@ -290,28 +278,16 @@ def test_c3_batch_certificate(device):
     ws = run(device, dg, batch, bfn.JACOBI_PULL, bfn.PERSISTENT, 0)
     st, per, _ = ws.status()
     assert st.converged == 1
-    # The tuned configuration: frontier push, LDS boundary proposals,
+    # The tuned configuration: frontier push, direct boundary atomics,
     # hipGraph replay of the rounds.
     tuned = run(device, dg, batch, bfn.FRONTIER_PUSH, bfn.CHUNKED, 0, block=256,
                 items_per_claim=2, use_graph=1, graph_rounds=36, boundary_mode=1)
     assert tuned.status()[0].converged == 1
-    windowed = run(device, dg, batch, bfn.FRONTIER_PUSH, bfn.CHUNKED, 0, block=256,
-                   items_per_claim=2, use_graph=1, graph_rounds=48, boundary_mode=1, window=2.0)
-    assert windowed.status()[0].converged == 1
-    # One workgroup per CU: each workgroup visits ~100 tiles per round, more
-    # than its LDS holds of deferred next-round enqueues (the overflow path).
-    narrow = bfn.Workspace(device)
-    narrow.submit(dg, batch, bfn.run_options(engine=bfn.DENSE_CHAOTIC_PUSH, control_mode=bfn.CHUNKED,
-                                             block_size=256, grid_policy=1, blocks_per_wgp=1),
-                  bfn.tuning(tile_sweeps=0, boundary_mode=1))
-    assert narrow.status()[0].converged == 1
     sample = list(range(0, 256, 16)) + [i for i, q in enumerate(qs) if q.unbounded][:2]
     for i in sample:
         d = ws.distances(i)
         assert certify_sssp(lay, [qs[i].source], tiles[i], d) == "", i
         assert np.array_equal(bits(tuned.distances(i)), bits(d)), i
-        assert np.array_equal(bits(windowed.distances(i)), bits(d)), i
-        assert np.array_equal(bits(narrow.distances(i)), bits(d)), i
     assert [p["distance"] for p in tuned.paths()] == [p["distance"] for p in ws.paths()]
     view = dict(vertex_count=lay["vertex_count"], tile_count=lay["tile_count"],
                 row_offsets=lay["csr_row_offsets"].astype(np.uint64),
