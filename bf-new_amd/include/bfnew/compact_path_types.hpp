@@ -61,4 +61,35 @@ struct CompactTransferAccounting {
   constexpr bool operator==(const CompactTransferAccounting&) const noexcept = default;
 };
 
+// One query's compact paths: its targets in canonical (sorted) order and the
+// map from the caller's target terminals to them
+// (reference: include/bfnew/compact_path_types.hpp:65-72).
+struct CompactPathPayload {
+  QueryId query_id{};
+  std::uint32_t expansion_generation{};
+  std::vector<std::uint32_t> target_terminal_to_target;
+  std::vector<CompactTargetPath> targets;
+  bool operator==(const CompactPathPayload&) const = default;
+};
+
+enum class CompactStageTimingEvidence : std::uint32_t {
+  unavailable = 0U,
+  measured = 1U,
+};
+
+// Device-event and host-wall stage durations of one compact-path execution
+// (reference: include/bfnew/compact_path_types.hpp:96-111).
+struct CompactPathExecutionMetrics {
+  CompactStageTimingEvidence device_timing{CompactStageTimingEvidence::unavailable};
+  CompactStageTimingEvidence host_timing{CompactStageTimingEvidence::unavailable};
+  std::uint64_t sssp_device_nanoseconds{};
+  std::uint64_t reconstruction_device_nanoseconds{};
+  std::uint64_t result_transfer_device_nanoseconds{};
+  std::uint64_t sssp_host_nanoseconds{};
+  std::uint64_t reconstruction_host_nanoseconds{};
+  std::uint64_t result_transfer_host_nanoseconds{};
+  std::uint64_t end_to_end_host_nanoseconds{};
+  constexpr bool operator==(const CompactPathExecutionMetrics&) const noexcept = default;
+};
+
 }  // namespace bfnew

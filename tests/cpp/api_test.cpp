@@ -16,6 +16,7 @@
 #include "bfnew/batch_layout.hpp"
 #include "bfnew/batch_plan.hpp"
 #include "bfnew/gpu_api.hpp"
+#include "bfnew/hip/batched_expansion.hpp"
 #include "bfnew/hip/engines.hpp"
 #include "bfnew/hip/runtime.hpp"
 #include "bfnew/query.hpp"
@@ -281,6 +282,77 @@ int gpu(const char* out_path) {
   const hip::CompactPathBatchOutput paths =
       dense.run_compact_paths(batch, options_for(EngineKind::dense_chaotic_push), pws);
   CHECK(paths.targets.size() == batch.targets.size());
+
+  // The expansion runner boundary: the planner's lane batch through
+  // BatchedExpansionExecutor, in each transfer mode, equals the engine run
+  // directly on the same planned description.
+  {
+    std::vector<RouteQuery> by_id = qs;
+    std::sort(by_id.begin(), by_id.end(),
+              [](const RouteQuery& x, const RouteQuery& y) { return x.query_id < y.query_id; });
+    const SelectedRegionIndex regions{g, runs};
+    const auto features = make_batch_query_features(g, regions, by_id);
+    BatchPlannerPolicy policy;
+    policy.lane_width = 8U;
+    policy.minimum_jaccard_numerator = 0U;  // one batch of all six
+    policy.maximum_union_inflation_numerator = 64U;
+    const BatchPlan plan = make_overlapping_batch_plan(regions, features, policy);
+    CHECK(plan.batches.size() == 1U && validate_batch_plan(regions, features, plan).ok());
+    const BatchPlanEntry& entry = plan.batches[0];
+    BatchDeviceDescription planned;
+    prepare_batch_device_description(g, by_id, features, entry, planned);
+    const ExpansionBatchContext ctx{};
+    for (EngineKind kind : {EngineKind::jacobi_pull, EngineKind::dense_chaotic_push,
+                            EngineKind::frontier_push}) {
+      const GpuRunOptions o = options_for(kind);
+      hip::BatchedExpansionExecutor status_exec(g, runs, resident, ws, stream, o);
+      CHECK(status_exec.kind() == kind);
+      const ExpansionBatchExecution st = status_exec(by_id, features, entry, ctx);
+      const DeviceRunStatus direct =
+          kind == EngineKind::jacobi_pull ? jacobi.run_compact_status(planned, o)
+          : kind == EngineKind::dense_chaotic_push ? dense.run_compact_status(planned, o)
+                                                    : frontier.run_compact_status(planned, o);
+      CHECK(st.result.status.converged == 1U && st.result.status.valid_lane_mask == 0x3FU);
+      CHECK(st.result.status.reached_target_mask == direct.reached_target_mask);
+      CHECK(st.work_evidence == ExpansionWorkEvidence::unavailable && st.compact_paths.empty());
+
+      hip::BatchedExpansionExecutor work_exec(g, runs, resident, ws, stream, o,
+                                              hip::BatchedExpansionTransferMode::status_and_work_evidence);
+      const ExpansionBatchExecution wk = work_exec(by_id, features, entry, ctx);
+      CHECK(wk.work_evidence == ExpansionWorkEvidence::measured);
+      CHECK(wk.shared_edge_work > 0U && wk.logical_lane_edge_work > 0U);
+      CHECK(wk.result.status.reached_target_mask == direct.reached_target_mask);
+
+      hip::BatchedExpansionExecutor path_exec(g, runs, resident, ws, stream, o,
+                                              hip::BatchedExpansionTransferMode::compact_paths);
+      const ExpansionBatchExecution pe = path_exec(by_id, features, entry, ctx);
+      const hip::CompactPathBatchOutput dp =
+          kind == EngineKind::jacobi_pull ? jacobi.run_compact_paths(planned, o, pws)
+          : kind == EngineKind::dense_chaotic_push ? dense.run_compact_paths(planned, o, pws)
+                                                    : frontier.run_compact_paths(planned, o, pws);
+      CHECK(pe.compact_paths.size() == 6U);
+      CHECK(pe.compact_transfer == dp.transfer);
+      for (std::size_t i = 1; i < pe.compact_paths.size(); ++i)
+        CHECK(pe.compact_paths[i - 1].query_id < pe.compact_paths[i].query_id);
+      for (std::uint32_t lane = 0; lane < 6; ++lane) {
+        const QueryId id = entry.query_ids_by_lane[lane];
+        const auto it = std::find_if(pe.compact_paths.begin(), pe.compact_paths.end(),
+                                     [&](const CompactPathPayload& p) { return p.query_id == id; });
+        CHECK(it != pe.compact_paths.end());
+        if (it == pe.compact_paths.end()) continue;
+        const auto b0 = dp.targets.begin() + planned.target_offsets[lane];
+        const auto b1 = dp.targets.begin() + planned.target_offsets[lane + 1];
+        CHECK(std::equal(it->targets.begin(), it->targets.end(), b0, b1));
+        CHECK(it->target_terminal_to_target == by_id[entry.query_indices_by_lane[lane]].target_terminal_to_target);
+      }
+    }
+    GpuRunOptions traced = options_for(EngineKind::frontier_push);
+    traced.instrumentation = InstrumentationLevel::light;
+    CHECK(throws<std::invalid_argument>([&] {
+      hip::BatchedExpansionExecutor bad(g, runs, resident, ws, stream, traced,
+                                        hip::BatchedExpansionTransferMode::compact_paths);
+    }));
+  }
 
   // Single-query engines reproduce each lane.
   hip::ReusableDeviceWorkspace single_ws(stream);
