@@ -19,10 +19,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "bf-new_amd", "lib", "bfnew_api_test")
 
 
-def _run(*args, timeout=120):
+def _run(*args, timeout=120, env=None):
     if not os.path.exists(DRIVER):
         pytest.fail("bf-new_amd/lib/bfnew_api_test is not built (run __graft_entry__.build())")
-    p = subprocess.run([DRIVER, *args], capture_output=True, text=True, timeout=timeout)
+    p = subprocess.run([DRIVER, *args], capture_output=True, text=True, timeout=timeout, env=env)
     assert p.returncode == 0, p.stdout + p.stderr
     return p
 
@@ -75,11 +75,10 @@ def _read(path):
     return V, W, lanes, dist, paths
 
 
-@pytest.mark.gpu
-def test_cpp_api_engines_match_oracle(tmp_path):
-    out = tmp_path / "api.bin"
-    _run("gpu", str(out))
-    V, W, lanes, dist, paths = _read(out)
+def check_api_output(path):
+    """The lanes, labels and paths the driver's GPU mode wrote, against the
+    oracle on the 16x16x64 graph (seed 1)."""
+    V, W, lanes, dist, paths = _read(path)
     hg = bfn.HostGraph.synthetic(16, 16, 64, 1, host_graph=True)
     view = hg.view()
     assert view["vertex_count"] == V
@@ -99,3 +98,10 @@ def test_cpp_api_engines_match_oracle(tmp_path):
             assert np.array_equal(labels.view(np.uint32), ref[verts].view(np.uint32))
         else:
             assert status == 1 and len(verts) == 0
+
+
+@pytest.mark.gpu
+def test_cpp_api_engines_match_oracle(tmp_path):
+    out = tmp_path / "api.bin"
+    _run("gpu", str(out))
+    check_api_output(out)
