@@ -185,7 +185,9 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
       for (int j = 0; j < EPT; ++j)
         if (internal.pack[j] != kNone) relax_staged(internal.pack[j], internal.w[j]);
     }
-    regs_key = resident ? key : kNone;
+    // A tile without internal edges loads nothing: the registers still hold
+    // the records regs_key names, which must not be taken for T's.
+    if (ei > e0) regs_key = resident ? key : kNone;
   }
   st.mark(1);
   for (std::uint32_t base = ei; base < e1; base += B * stream_records(B)) {
