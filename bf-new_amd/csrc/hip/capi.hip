@@ -177,6 +177,7 @@ struct bfn_workspace_s {
   std::size_t lds_bytes = 0;
   const unsigned long long* frame_base_ptr = nullptr;  // in `meta`
   bool submitted = false, completed = false, have_paths = false;
+  bool paths_counted = false;  // pass 1 of this completion's paths is on the host
   std::uint64_t arena_vertices = 0;
   std::uint64_t launches = 0, host_checks = 0, host_syncs = 0, controller_copies = 0;
   hipEvent_t ev_start = nullptr, ev_relax0 = nullptr, ev_relax1 = nullptr, ev_stop = nullptr;
@@ -754,6 +755,7 @@ int32_t run_resident_batch(bfn_workspace ws) {
   const DevBatch& b = ws->view;
   ws->completed = false;
   ws->have_paths = false;
+  ws->paths_counted = false;
   BFN_HIP(hipEventRecord(ws->ev_start, s));
   const bool graph_mode = t.use_graph && o.control_mode != 0;
   unsigned long long* relax = t.count_relaxations ? ws->relax.as<unsigned long long>() : nullptr;
@@ -1114,11 +1116,6 @@ int32_t bfn_batch_paths(bfn_workspace ws, bfn_target_summary* summaries, uint64_
   BFN_HIP(ws->clean.reserve(4ull * ws->Q));
   BFN_HIP(ws->h_paths.reserve(28ull * NT + 8ull * (NT + 1) + 4ull * ws->Q));
   char* hp = ws->h_paths.as<char>();
-  std::uint32_t* hclean = reinterpret_cast<std::uint32_t*>(hp + 28ull * NT + 8ull * (NT + 1));
-  for (std::uint32_t q = 0; q < ws->Q; ++q)
-    hclean[q] = (qs[q].converged && qs[q].error_bits == 0) ? 1u : 0u;
-  BFN_HIP(hipMemcpyAsync(ws->clean.ptr, hclean, 4ull * ws->Q, hipMemcpyHostToDevice, s));
-  BFN_HIP(fill_launch(ws->onpath.as<std::uint32_t>(), ws->pool_words, 0u, s));
   PathArgs a{};
   a.g = ws->graph->view;
   a.b = ws->view;
@@ -1134,11 +1131,21 @@ int32_t bfn_batch_paths(bfn_workspace ws, bfn_target_summary* summaries, uint64_
   a.summaries = ws->paths_summary.as<std::uint32_t>();
   a.lengths = ws->paths_len.as<unsigned long long>();
   a.offsets = ws->paths_off.as<unsigned long long>();
-  BFN_HIP(paths_launch(a, ws->Q, s));
-  BFN_HIP(scan_u64_launch(a.lengths, ws->paths_off.as<unsigned long long>(), NT, s));
-  BFN_HIP(hipMemcpyAsync(hp, ws->paths_summary.ptr, 28ull * NT, hipMemcpyDeviceToHost, s));
-  BFN_HIP(hipMemcpyAsync(hp + 28ull * NT, ws->paths_off.ptr, 8ull * (NT + 1), hipMemcpyDeviceToHost, s));
-  BFN_HIP(hipStreamSynchronize(s));
+  // Pass 1 (summaries, lengths, offsets) runs once per completed batch: a
+  // second call, the one that brings the arenas, reuses its results.
+  if (!ws->paths_counted) {
+    std::uint32_t* hclean = reinterpret_cast<std::uint32_t*>(hp + 28ull * NT + 8ull * (NT + 1));
+    for (std::uint32_t q = 0; q < ws->Q; ++q)
+      hclean[q] = (qs[q].converged && qs[q].error_bits == 0) ? 1u : 0u;
+    BFN_HIP(hipMemcpyAsync(ws->clean.ptr, hclean, 4ull * ws->Q, hipMemcpyHostToDevice, s));
+    BFN_HIP(fill_launch(ws->onpath.as<std::uint32_t>(), ws->pool_words, 0u, s));
+    BFN_HIP(paths_launch(a, ws->Q, s));
+    BFN_HIP(scan_u64_launch(a.lengths, ws->paths_off.as<unsigned long long>(), NT, s));
+    BFN_HIP(hipMemcpyAsync(hp, ws->paths_summary.ptr, 28ull * NT, hipMemcpyDeviceToHost, s));
+    BFN_HIP(hipMemcpyAsync(hp + 28ull * NT, ws->paths_off.ptr, 8ull * (NT + 1), hipMemcpyDeviceToHost, s));
+    BFN_HIP(hipStreamSynchronize(s));
+    ws->paths_counted = true;
+  }
   std::memcpy(summaries, hp, 28ull * NT);
   std::memcpy(path_offsets, hp + 28ull * NT, 8ull * (NT + 1));
   const std::uint64_t total = path_offsets[NT];

@@ -62,27 +62,30 @@ class BatchedExpansionExecutor::Impl {
     switch (mode_) {
       case BatchedExpansionTransferMode::compact_status:
         out.result.status = compact_status();
-        out.compact_status_bytes = sizeof(DeviceRunStatus);
         break;
       case BatchedExpansionTransferMode::status_and_work_evidence: {
         GpuRunOptions o = options_;
         if (o.instrumentation == InstrumentationLevel::none) o.instrumentation = InstrumentationLevel::light;
         const BatchedRunOutput r = status_only(o);
         out.result = r.result;
+        // A tile visit serves one (query, tile) slot, so every edge record
+        // it reads is relaxed for exactly one lane: logical work equals the
+        // records the visits read.
         out.work_evidence = ExpansionWorkEvidence::measured;
         out.shared_edge_work = r.metrics.edge_records_evaluated;
-        for (std::uint64_t n : r.lane_relaxations) out.logical_lane_edge_work += n;
-        out.compact_status_bytes = sizeof(DeviceRunStatus);
+        out.logical_lane_edge_work = r.metrics.edge_records_evaluated;
         break;
       }
       case BatchedExpansionTransferMode::compact_paths:
         compact_paths(queries, out);
         break;
     }
-    out.compact_total_device_to_host_bytes =
-        out.compact_transfer.total_bytes + out.compact_status_bytes + out.compact_error_bytes;
-    out.compact_overall_device_to_host_bytes =
-        out.compact_total_device_to_host_bytes + out.compact_controller_poll_bytes;
+    if (mode_ == BatchedExpansionTransferMode::compact_paths) {
+      out.compact_total_device_to_host_bytes =
+          out.compact_transfer.total_bytes + out.compact_status_bytes + out.compact_error_bytes;
+      out.compact_overall_device_to_host_bytes =
+          out.compact_total_device_to_host_bytes + out.compact_controller_poll_bytes;
+    }
     return out;
   }
 
@@ -111,23 +114,27 @@ class BatchedExpansionExecutor::Impl {
 
   // One payload per valid lane of a cleanly converged batch, sorted by id;
   // the targets of lane l are description_.targets[target_offsets[l] ..).
+  // A batch that stopped otherwise returns no payload and no byte record.
   void compact_paths(std::span<const RouteQuery> queries, ExpansionBatchExecution& out) {
     CompactPathBatchOutput r = paths();
     out.result.status = r.status;
-    out.compact_transfer = r.transfer;
-    out.compact_status_bytes = sizeof(DeviceRunStatus);
-    out.compact_execution.device_timing = CompactStageTimingEvidence::measured;
-    out.compact_execution.sssp_device_nanoseconds = nanoseconds(r.metrics.sssp_device_milliseconds);
-    out.compact_execution.reconstruction_device_nanoseconds =
-        nanoseconds(r.metrics.reconstruction_device_milliseconds);
-    out.compact_execution.end_to_end_host_nanoseconds =
-        nanoseconds(r.metrics.end_to_end_wall_milliseconds);
+    CompactPathExecutionMetrics& t = out.compact_execution;
+    t.host_timing = CompactStageTimingEvidence::measured;
+    t.sssp_host_nanoseconds = nanoseconds(r.metrics.sssp_host_milliseconds);
+    t.reconstruction_host_nanoseconds = nanoseconds(r.metrics.reconstruction_host_milliseconds);
+    // The path passes and their copies are timed together, as reconstruction.
+    t.end_to_end_host_nanoseconds = std::max(nanoseconds(r.metrics.end_to_end_wall_milliseconds),
+                                             t.sssp_host_nanoseconds + t.reconstruction_host_nanoseconds);
     const BatchDeviceDescription& d = description_;
     if (r.targets.size() != d.targets.size())
       throw std::runtime_error{"compact paths disagree with the batch's targets"};
-    const bool clean = r.status.stop_reason == static_cast<std::uint32_t>(DeviceStopReason::converged) &&
+    const bool clean = r.status.converged == 1U &&
+                       r.status.stop_reason == static_cast<std::uint32_t>(DeviceStopReason::converged) &&
                        r.status.error_bits == 0U;
     if (!clean) return;
+    out.compact_transfer = r.transfer;
+    out.compact_status_bytes = sizeof(DeviceRunStatus);
+    out.compact_error_bytes = r.metrics.control_bytes;
     for (std::uint32_t lane = 0; lane < d.lane_width; ++lane) {
       if (!(d.valid_lane_mask >> lane & 1U)) continue;
       const RouteQuery& q =

@@ -286,8 +286,12 @@ Completed submit_and_wait(bfn_workspace ws, bfn_graph graph, const bfn_batch_des
   return c;
 }
 
+// Both path passes; `transfer` counts the payload as it crosses the bus
+// (28-byte summaries, 4-byte vertices, labels and edge ids) and
+// `control_bytes` the offsets and edge-arena slots that travel beside it.
 std::vector<CompactTargetPath> read_paths(bfn_workspace ws, std::size_t target_count,
-                                          CompactTransferAccounting& transfer) {
+                                          CompactTransferAccounting& transfer,
+                                          std::uint64_t& control_bytes) {
   std::vector<bfn_target_summary> summaries(target_count);
   std::vector<std::uint64_t> offsets(target_count + 1U);
   std::uint64_t total = 0;
@@ -319,11 +323,13 @@ std::vector<CompactTargetPath> read_paths(bfn_workspace ws, std::size_t target_c
     }
     transfer.vertex_bytes += 4U * p.vertices.size();
     transfer.distance_label_bytes += 4U * p.distance_labels.size();
-    transfer.edge_id_bytes += 8U * p.edge_ids.size();
+    transfer.edge_id_bytes += 4U * p.edge_ids.size();
   }
   transfer.summary_bytes = sizeof(bfn_target_summary) * target_count;
   transfer.total_bytes = transfer.summary_bytes + transfer.vertex_bytes +
                          transfer.distance_label_bytes + transfer.edge_id_bytes;
+  // The edge arena is copied at the vertex arena's length.
+  control_bytes = 8U * (target_count + 1U) + (total ? 4U * total - transfer.edge_id_bytes : 0U);
   return out;
 }
 
@@ -411,10 +417,12 @@ BatchedRunOutput BatchedEngineBase::run_impl(const BatchDeviceDescription& batch
   if (readback == Readback::compact_paths && paths) {
     paths->status = c.status;
     const auto t0 = Clock::now();
-    paths->targets = read_paths(workspace_.handle(), lanes.targets.size(), paths->transfer);
+    paths->targets = read_paths(workspace_.handle(), lanes.targets.size(), paths->transfer,
+                                paths->metrics.control_bytes);
+    const auto t1 = Clock::now();
     paths->metrics.sssp_device_milliseconds = total_ms;
-    paths->metrics.reconstruction_device_milliseconds =
-        std::chrono::duration<float, std::milli>(Clock::now() - t0).count();
+    paths->metrics.sssp_host_milliseconds = std::chrono::duration<double, std::milli>(t0 - wall0).count();
+    paths->metrics.reconstruction_host_milliseconds = std::chrono::duration<double, std::milli>(t1 - t0).count();
     paths->metrics.end_to_end_wall_milliseconds =
         std::chrono::duration<double, std::milli>(Clock::now() - wall0).count();
   }

@@ -69,9 +69,13 @@ using BatchedDenseChaoticPushRunOutput = BatchedRunOutput;
 using BatchedFrontierPushRunOutput = BatchedRunOutput;
 
 struct CompactPathBatchMetrics {
-  float sssp_device_milliseconds{};
-  float reconstruction_device_milliseconds{};
+  float sssp_device_milliseconds{};            // HIP events around the batch's kernels
+  double sssp_host_milliseconds{};             // host wall: submit to completion
+  double reconstruction_host_milliseconds{};   // host wall: both path passes and their copies
   double end_to_end_wall_milliseconds{};
+  // Device-to-host bytes beside the payload: the per-target path offsets and
+  // the edge arena's one unused slot per path.
+  std::uint64_t control_bytes{};
 };
 
 struct CompactPathBatchOutput {

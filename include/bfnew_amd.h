@@ -274,7 +274,9 @@ int32_t bfn_batch_predecessors(bfn_workspace workspace, uint32_t query, uint32_t
  * canonical target order); path_offsets[i]..path_offsets[i+1] index the
  * vertex and label arenas; target i's path_length edges start at
  * path_offsets[i] in the edge arena (which has the vertex arena's size).
- * Pass NULL arenas to learn *vertex_total first. */
+ * Pass NULL arenas to learn *vertex_total first; the summaries and offsets
+ * are computed once per completed batch, so the second call, with the
+ * arenas, only emits the paths. */
 int32_t bfn_batch_paths(bfn_workspace workspace, bfn_target_summary *summaries,
                         uint64_t *path_offsets, uint32_t *vertices, float *labels,
                         uint32_t *edge_ids, uint64_t arena_capacity, uint64_t *vertex_total);

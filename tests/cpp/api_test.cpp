@@ -15,6 +15,7 @@
 // batched engine tests; the cases below are this library's own.
 #include "bfnew/batch_layout.hpp"
 #include "bfnew/batch_plan.hpp"
+#include "bfnew/compact_paths.hpp"
 #include "bfnew/gpu_api.hpp"
 #include "bfnew/hip/batched_expansion.hpp"
 #include "bfnew/hip/engines.hpp"
@@ -332,6 +333,14 @@ int gpu(const char* out_path) {
                                                     : frontier.run_compact_paths(planned, o, pws);
       CHECK(pe.compact_paths.size() == 6U);
       CHECK(pe.compact_transfer == dp.transfer);
+      CHECK(pe.compact_transfer == measure_compact_transfer(pe.compact_paths));
+      CHECK(pe.compact_execution.host_timing == CompactStageTimingEvidence::measured);
+      CHECK(pe.compact_total_device_to_host_bytes ==
+            pe.compact_transfer.total_bytes + sizeof(DeviceRunStatus) + pe.compact_error_bytes);
+      for (const CompactPathPayload& p : pe.compact_paths)
+        CHECK(validate_compact_path_payload(g, *std::find_if(by_id.begin(), by_id.end(), [&](const RouteQuery& q) {
+                return q.query_id == p.query_id;
+              }), p).ok());
       for (std::size_t i = 1; i < pe.compact_paths.size(); ++i)
         CHECK(pe.compact_paths[i - 1].query_id < pe.compact_paths[i].query_id);
       for (std::uint32_t lane = 0; lane < 6; ++lane) {
