@@ -86,12 +86,17 @@ def make_batch(hg, cfg):
     return qs, bfn.Batch([[q.source] for q in qs], [[q.target] for q in qs], tiles)
 
 
-def apply_tuning(batch, qs, tune_entry):
+def tuned_defaults(tune_entry):
     t = dict(engine=bfn.JACOBI_PULL, block=256, sweeps=0, control=bfn.PERSISTENT,
              classes=None, items_per_claim=1, use_graph=0, graph_rounds=0, boundary_mode=0,
              min_waves=0)
     if tune_entry:
         t.update(tune_entry)
+    return t
+
+
+def apply_tuning(batch, qs, tune_entry):
+    t = tuned_defaults(tune_entry)
     if t.get("classes"):
         batch.engines = [int(t["classes"][query_class(q)]) for q in qs]
     return t
@@ -319,6 +324,29 @@ def copy_calibration():
     log("copy calibration: 3 x 1073741824 bytes copied")
 
 
+EXPANSION_DRIVER = os.path.join(ROOT, "bf-new_amd", "lib", "bfnew_expansion_bench")
+
+
+def expansion(tune):
+    """run_batched_expansion over the device executor (compact paths) on the
+    64x64x256 graph, 256 queries starting from their two terminal tiles; one
+    record per expansion schedule (bf-new_amd/csrc/bench/expansion_bench.cpp)."""
+    import subprocess
+    cmd = [EXPANSION_DRIVER, "--tiles", "64", "64", "256", "--queries", "256",
+           "--engine", str(tune["engine"]), "--control", str(bfn.CHUNKED),
+           "--sweeps", str(tune["sweeps"]), "--boundary", str(tune["boundary_mode"]),
+           "--graph", str(tune["use_graph"]), "--graph-rounds", "8",
+           "--claim", str(tune["items_per_claim"]), "--min-waves", str(tune["min_waves"])]
+    log("expansion:", " ".join(cmd[1:]))
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    sys.stderr.write(p.stderr)
+    if p.returncode != 0:
+        raise SystemExit(f"expansion driver failed ({p.returncode})")
+    out = json.loads(p.stdout.strip().splitlines()[-1])
+    log("expansion:", json.dumps(out))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -331,6 +359,8 @@ def main():
                     help="also run the negotiated-cost rounds on the 256x256x512 graph")
     ap.add_argument("--serving", action="store_true",
                     help="also run the Poisson-arrival serving measurement")
+    ap.add_argument("--expansion", action="store_true",
+                    help="also time the expansion orchestrator over the device executor")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--engine", type=int, default=None, help="override the tuned engine")
@@ -344,6 +374,9 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
+    # The expansion driver is its own process; it runs before this one opens
+    # the device.
+    expansion_result = expansion(tuned_defaults(load_tuning(C3["name"]))) if args.expansion else None
     dev = bfn.Device(0)
     info = dev.info()
     log("device", info.arch.decode(), info.compute_units, "CUs")
@@ -444,6 +477,8 @@ def main():
         result["serving"] = serving(dev)
     if args.negotiated:
         result["negotiated"] = negotiated(dev, dict(tune, control=bfn.PERSISTENT))
+    if expansion_result is not None:
+        result["expansion"] = expansion_result
     if not args.no_cpu_baseline:
         v = hg.layout_arrays()
         view = dict(vertex_count=v["vertex_count"], tile_count=v["tile_count"],

@@ -52,3 +52,30 @@ def test_expansion_orchestrator_emulated_executor():
 @pytest.mark.gpu
 def test_expansion_orchestrator_executor():
     _run("gpu")
+
+
+def test_expansion_bench_driver_emulated():
+    """The bench driver (bench.py --expansion) end to end on the CPU emulation,
+    at a size the emulation finishes in seconds: every query is reached, the
+    misses of the terminal-tiles start expand once, and the records add up."""
+    import json
+    sys.path.insert(0, os.path.join(HERE, "emu"))
+    import build_emu
+    lib = build_emu.build()
+    env = dict(os.environ)
+    env["LD_LIBRARY_PATH"] = os.path.dirname(lib) + (
+        ":" + env["LD_LIBRARY_PATH"] if env.get("LD_LIBRARY_PATH") else "")
+    env.update(EMU_CUS="2", EMU_OCC="2", EMU_YIELD_PCT="0")
+    driver = os.path.join(ROOT, "bf-new_amd", "lib", "bfnew_expansion_bench")
+    p = subprocess.run([driver, "--tiles", "8", "8", "16", "--queries", "24", "--lane-width", "8",
+                        "--reps", "1"], capture_output=True, text=True, timeout=900, env=env)
+    assert p.returncode == 0, p.stdout + p.stderr
+    out = json.loads(p.stdout.strip().splitlines()[-1])
+    assert out["workload"] == "expansion_8x8x16_q24" and len(out["schedules"]) == 4
+    for row in out["schedules"]:
+        assert row["reached"] == 24 and row["other_failures"] == 0
+        assert sum(row["histogram"]) == 24
+        assert row["scheduled_expansions"] + row["full_region_fallbacks"] == sum(
+            i * n for i, n in enumerate(row["histogram"]))
+        assert row["initial_reached"] + sum(row["histogram"][1:]) == 24
+        assert row["path_vertices"] > 0 and row["compact_d2h_bytes"] > 0
