@@ -102,52 +102,146 @@ void require_plannable(const SelectedRegionIndex& index, std::span<const BatchQu
   if (!ids_unique(queries)) throw std::invalid_argument{"batch planner query ids must be unique"};
 }
 
-// The ranking key of a query joining the current batch; smaller ranks first.
+// The open batch: lanes, union tiles (sorted, and as a membership map), the
+// lanes' terminal tiles, and the union's running vertex and edge counts.
+struct OpenBatch {
+  std::vector<std::uint32_t> lanes;
+  std::vector<TileId> tiles, source_tiles, target_tiles;
+  std::uint64_t lane_tiles = 0;  // selected tile positions of the lanes
+  std::uint64_t vertices = 0, edges = 0;
+};
+
+// One query's offer to join the open batch, ranked by (larger Jaccard, more
+// shared terminal tiles, smaller inflation, fewer union edges, fewer union
+// vertices, fewer new tiles, closer generation / source / target counts to
+// the anchor, smaller id).  The union's edge and vertex counts are costly, so
+// they are computed only when a comparison reaches them.
 struct Offer {
   std::uint32_t query = 0;
   BatchFraction jaccard{};
   std::uint64_t shared_terminals = 0;
   BatchFraction inflation{};
-  std::uint64_t union_edges = 0, union_vertices = 0, new_tiles = 0;
+  std::uint64_t new_tiles = 0;
   std::uint32_t generation_gap = 0, source_gap = 0, target_gap = 0, id = 0;
-  std::vector<TileId> union_tiles;
+  bool counted = false;
+  std::uint64_t union_edges = 0, union_vertices = 0;
+};
 
-  bool before(const Offer& o) const {
-    if (int c = order(jaccard, o.jaccard)) return c > 0;
-    if (shared_terminals != o.shared_terminals) return shared_terminals > o.shared_terminals;
-    if (int c = order(inflation, o.inflation)) return c < 0;
-    return std::tie(union_edges, union_vertices, new_tiles, generation_gap, source_gap, target_gap,
-                    id) < std::tie(o.union_edges, o.union_vertices, o.new_tiles, o.generation_gap,
-                                   o.source_gap, o.target_gap, o.id);
+// Tile-set arithmetic against the open batch: `in_union` marks the union's
+// tiles and `common[q]` holds |union n selected(q)| for every query.
+class Planner {
+ public:
+  Planner(const SelectedRegionIndex& index, std::span<const BatchQueryFeatures> queries)
+      : index_{index}, queries_{queries}, in_union_(index.tile_count(), 0U), in_new_(index.tile_count(), 0U),
+        common_(queries.size(), 0U), holders_offsets_(index.tile_count() + 1U, 0U) {
+    for (const auto& f : queries)
+      for (TileId t : f.selected_tiles) ++holders_offsets_[t.value() + 1U];
+    for (std::size_t t = 0; t < index.tile_count(); ++t) holders_offsets_[t + 1] += holders_offsets_[t];
+    holders_.resize(holders_offsets_.back());
+    std::vector<std::uint64_t> fill(holders_offsets_.begin(), holders_offsets_.end() - 1);
+    for (std::uint32_t q = 0; q < queries.size(); ++q)
+      for (TileId t : queries[q].selected_tiles) holders_[fill[t.value()]++] = q;
   }
-};
 
-struct OpenBatch {
-  std::vector<std::uint32_t> lanes;
-  std::vector<TileId> tiles, source_tiles, target_tiles;
-  std::uint64_t lane_tiles = 0;  // selected tile positions of the lanes
-};
+  void open(OpenBatch& b, std::uint32_t anchor) {
+    const BatchQueryFeatures& f = queries_[anchor];
+    b.lanes = {anchor};
+    b.tiles = f.selected_tiles;
+    b.source_tiles = f.source_tiles;
+    b.target_tiles = f.target_tiles;
+    b.lane_tiles = f.selected_tiles.size();
+    b.vertices = f.selected_vertex_count;
+    b.edges = f.selected_edge_estimate;
+    for (TileId t : b.tiles) mark(t.value());
+  }
 
-Offer make_offer(const SelectedRegionIndex& index, std::span<const BatchQueryFeatures> queries,
-                 std::uint32_t q, const BatchQueryFeatures& anchor, const OpenBatch& batch) {
-  const BatchQueryFeatures& f = queries[q];
-  Offer o;
-  o.query = q;
-  o.jaccard = tile_set_jaccard(batch.tiles, f.selected_tiles);
-  o.shared_terminals = common_count(batch.source_tiles, f.source_tiles) +
-                       common_count(batch.target_tiles, f.target_tiles);
-  o.union_tiles = merged(batch.tiles, f.selected_tiles);
-  o.inflation = BatchFraction{(batch.lanes.size() + 1U) * o.union_tiles.size(),
-                              batch.lane_tiles + f.selected_tiles.size()};
-  o.union_edges = index.selected_edge_count(o.union_tiles);
-  o.union_vertices = index.selected_vertex_count(o.union_tiles);
-  o.new_tiles = o.union_tiles.size() - batch.tiles.size();
-  o.generation_gap = gap(f.expansion_generation, anchor.expansion_generation);
-  o.source_gap = gap(f.source_count, anchor.source_count);
-  o.target_gap = gap(f.target_count, anchor.target_count);
-  o.id = f.query_id.value();
-  return o;
-}
+  void close(const OpenBatch& b) {
+    for (TileId t : b.tiles) {
+      in_union_[t.value()] = 0U;
+      for (auto i = holders_offsets_[t.value()]; i < holders_offsets_[t.value() + 1]; ++i) common_[holders_[i]] = 0U;
+    }
+  }
+
+  Offer offer(const OpenBatch& b, std::uint32_t q, const BatchQueryFeatures& anchor) const {
+    const BatchQueryFeatures& f = queries_[q];
+    Offer o;
+    o.query = q;
+    const std::uint64_t both = common_[q];
+    const std::uint64_t either = b.tiles.size() + f.selected_tiles.size() - both;
+    o.jaccard = BatchFraction{both, either};
+    o.shared_terminals = common_count(b.source_tiles, f.source_tiles) + common_count(b.target_tiles, f.target_tiles);
+    o.inflation = BatchFraction{(b.lanes.size() + 1U) * either, b.lane_tiles + f.selected_tiles.size()};
+    o.new_tiles = f.selected_tiles.size() - both;
+    o.generation_gap = gap(f.expansion_generation, anchor.expansion_generation);
+    o.source_gap = gap(f.source_count, anchor.source_count);
+    o.target_gap = gap(f.target_count, anchor.target_count);
+    o.id = f.query_id.value();
+    return o;
+  }
+
+  // Union edges and vertices after admitting the offer's query: the union's
+  // own plus those of its new tiles N (edges N->N, N->union, union->N).
+  void count(const OpenBatch& b, Offer& o) {
+    if (o.counted) return;
+    const auto& sel = queries_[o.query].selected_tiles;
+    std::uint64_t vertices = b.vertices, edges = b.edges;
+    for (TileId t : sel)
+      if (!in_union_[t.value()]) in_new_[t.value()] = 1U;
+    for (TileId t : sel) {
+      const std::uint32_t n = t.value();
+      if (!in_new_[n]) continue;
+      vertices += index_.tile_vertex_count(n);
+      const auto out = index_.pairs_from(n);
+      for (std::size_t i = 0; i < out.tiles.size(); ++i)
+        if (in_union_[out.tiles[i]] || in_new_[out.tiles[i]]) edges += out.edges[i];
+      const auto in = index_.pairs_into(n);
+      for (std::size_t i = 0; i < in.tiles.size(); ++i)
+        if (in_union_[in.tiles[i]]) edges += in.edges[i];
+    }
+    for (TileId t : sel) in_new_[t.value()] = 0U;
+    o.union_vertices = vertices;
+    o.union_edges = edges;
+    o.counted = true;
+  }
+
+  bool before(const OpenBatch& b, Offer& x, Offer& y) {
+    if (int c = order(x.jaccard, y.jaccard)) return c > 0;
+    if (x.shared_terminals != y.shared_terminals) return x.shared_terminals > y.shared_terminals;
+    if (int c = order(x.inflation, y.inflation)) return c < 0;
+    count(b, x);
+    count(b, y);
+    return std::tie(x.union_edges, x.union_vertices, x.new_tiles, x.generation_gap, x.source_gap, x.target_gap,
+                    x.id) < std::tie(y.union_edges, y.union_vertices, y.new_tiles, y.generation_gap,
+                                     y.source_gap, y.target_gap, y.id);
+  }
+
+  void admit(OpenBatch& b, Offer& o) {
+    count(b, o);
+    const BatchQueryFeatures& f = queries_[o.query];
+    for (TileId t : f.selected_tiles)
+      if (!in_union_[t.value()]) mark(t.value());
+    b.lanes.push_back(o.query);
+    b.lane_tiles += f.selected_tiles.size();
+    b.tiles = merged(b.tiles, f.selected_tiles);
+    b.source_tiles = merged(b.source_tiles, f.source_tiles);
+    b.target_tiles = merged(b.target_tiles, f.target_tiles);
+    b.vertices = o.union_vertices;
+    b.edges = o.union_edges;
+  }
+
+ private:
+  void mark(std::uint32_t t) {
+    in_union_[t] = 1U;
+    for (auto i = holders_offsets_[t]; i < holders_offsets_[t + 1]; ++i) ++common_[holders_[i]];
+  }
+
+  const SelectedRegionIndex& index_;
+  std::span<const BatchQueryFeatures> queries_;
+  std::vector<std::uint8_t> in_union_, in_new_;
+  std::vector<std::uint32_t> common_;
+  std::vector<std::uint64_t> holders_offsets_;  // tile -> queries selecting it
+  std::vector<std::uint32_t> holders_;
+};
 
 BatchPlanEntry close_batch(const SelectedRegionIndex& index,
                            std::span<const BatchQueryFeatures> queries, std::uint32_t width,
@@ -200,37 +294,30 @@ BatchPlan plan_batches(const SelectedRegionIndex& index, std::span<const BatchQu
   const BatchFraction max_inflation{policy.maximum_union_inflation_numerator,
                                     policy.maximum_union_inflation_denominator};
   std::vector<bool> taken(queries.size(), false);
+  Planner planner{index, queries};
   for (std::uint32_t anchor_index : anchors) {
     if (taken[anchor_index]) continue;
     const BatchQueryFeatures& anchor = queries[anchor_index];
     taken[anchor_index] = true;
     OpenBatch batch;
-    batch.lanes = {anchor_index};
-    batch.tiles = anchor.selected_tiles;
-    batch.source_tiles = anchor.source_tiles;
-    batch.target_tiles = anchor.target_tiles;
-    batch.lane_tiles = anchor.selected_tiles.size();
+    planner.open(batch, anchor_index);
     while (batch.lanes.size() < policy.lane_width) {
       bool found = false;
       Offer best;
       for (std::uint32_t q = 0; q < queries.size(); ++q) {
         if (taken[q]) continue;
-        Offer o = make_offer(index, queries, q, anchor, batch);
+        Offer o = planner.offer(batch, q, anchor);
         if (order(o.jaccard, min_jaccard) < 0 || order(o.inflation, max_inflation) > 0) continue;
-        if (!found || o.before(best)) {
-          best = std::move(o);
+        if (!found || planner.before(batch, o, best)) {
+          best = o;
           found = true;
         }
       }
       if (!found) break;
-      const BatchQueryFeatures& joined = queries[best.query];
       taken[best.query] = true;
-      batch.lanes.push_back(best.query);
-      batch.lane_tiles += joined.selected_tiles.size();
-      batch.tiles = std::move(best.union_tiles);
-      batch.source_tiles = merged(batch.source_tiles, joined.source_tiles);
-      batch.target_tiles = merged(batch.target_tiles, joined.target_tiles);
+      planner.admit(batch, best);
     }
+    planner.close(batch);
     plan.batches.push_back(close_batch(index, queries, policy.lane_width, std::move(batch)));
   }
   return plan;
@@ -291,34 +378,56 @@ SelectedRegionIndex::SelectedRegionIndex(const WeightedGraph& graph,
   if (T > 0xFFFFFFFFull) throw std::overflow_error{"tile count exceeds TileId"};
   vertex_counts_.resize(T);
   for (std::size_t t = 0; t < T; ++t) vertex_counts_[t] = offsets[t + 1] - offsets[t];
-  // (source tile, destination tile, edges) of every run, coalesced per tile pair
-  struct PairCount {
-    std::uint32_t from, to;
-    std::uint64_t edges;
-  };
-  std::vector<PairCount> runs;
-  runs.reserve(tile_runs.csr_run_destination_tiles.size());
-  for (std::size_t v = 0; v < V; ++v)
-    for (auto r = tile_runs.csr_row_run_offsets[v]; r < tile_runs.csr_row_run_offsets[v + 1]; ++r) {
-      const std::uint32_t to = tile_runs.csr_run_destination_tiles[r].value();
-      if (to >= T || owners[v].value() >= T)
-        throw std::invalid_argument{"tile run names a tile outside the graph"};
-      runs.push_back({owners[v].value(), to,
-                      tile_runs.csr_run_edge_offsets[r + 1] - tile_runs.csr_run_edge_offsets[r]});
-    }
-  std::sort(runs.begin(), runs.end(), [](const PairCount& x, const PairCount& y) {
-    return std::tie(x.from, x.to) < std::tie(y.from, y.to);
-  });
+  // (source tile, destination tile) edge counts, coalesced per pair.  Rows
+  // are tile-major, so each source tile's runs are contiguous: accumulate
+  // them into a dense per-destination slot and emit the touched slots.
+  std::vector<std::uint64_t> acc(T, 0U);
+  std::vector<std::uint32_t> touched;
   pair_offsets_.assign(T + 1U, 0U);
-  for (std::size_t i = 0; i < runs.size(); ++i) {
-    if (i == 0 || runs[i].from != runs[i - 1].from || runs[i].to != runs[i - 1].to) {
-      pair_tiles_.push_back(runs[i].to);
-      pair_edges_.push_back(0U);
-      ++pair_offsets_[runs[i].from + 1U];
+  for (std::size_t t = 0; t < T; ++t) {
+    for (std::size_t v = offsets[t]; v < offsets[t + 1]; ++v) {
+      if (owners[v].value() != t) throw std::invalid_argument{"selected-region index needs tile-major vertices"};
+      for (auto r = tile_runs.csr_row_run_offsets[v]; r < tile_runs.csr_row_run_offsets[v + 1]; ++r) {
+        const std::uint32_t to = tile_runs.csr_run_destination_tiles[r].value();
+        if (to >= T) throw std::invalid_argument{"tile run names a tile outside the graph"};
+        if (acc[to] == 0U) touched.push_back(to);
+        acc[to] += tile_runs.csr_run_edge_offsets[r + 1] - tile_runs.csr_run_edge_offsets[r];
+      }
     }
-    pair_edges_.back() += runs[i].edges;
+    std::sort(touched.begin(), touched.end());
+    for (std::uint32_t to : touched) {
+      pair_tiles_.push_back(to);
+      pair_edges_.push_back(acc[to]);
+      acc[to] = 0U;
+    }
+    pair_offsets_[t + 1] = pair_tiles_.size();
+    touched.clear();
   }
-  for (std::size_t t = 0; t < T; ++t) pair_offsets_[t + 1] += pair_offsets_[t];
+  // The transpose: pairs grouped by destination, sources ascending.
+  in_pair_offsets_.assign(T + 1U, 0U);
+  for (std::uint32_t to : pair_tiles_) ++in_pair_offsets_[to + 1U];
+  for (std::size_t t = 0; t < T; ++t) in_pair_offsets_[t + 1] += in_pair_offsets_[t];
+  in_pair_tiles_.resize(pair_tiles_.size());
+  in_pair_edges_.resize(pair_tiles_.size());
+  std::vector<std::uint64_t> fill(in_pair_offsets_.begin(), in_pair_offsets_.end() - 1);
+  for (std::size_t from = 0; from < T; ++from)
+    for (auto i = pair_offsets_[from]; i < pair_offsets_[from + 1]; ++i) {
+      const std::uint64_t at = fill[pair_tiles_[i]]++;
+      in_pair_tiles_[at] = static_cast<std::uint32_t>(from);
+      in_pair_edges_[at] = pair_edges_[i];
+    }
+}
+
+SelectedRegionIndex::TilePairs SelectedRegionIndex::pairs_from(std::uint32_t tile) const noexcept {
+  const std::size_t a = pair_offsets_[tile], b = pair_offsets_[tile + 1U];
+  return {std::span<const std::uint32_t>(pair_tiles_).subspan(a, b - a),
+          std::span<const std::uint64_t>(pair_edges_).subspan(a, b - a)};
+}
+
+SelectedRegionIndex::TilePairs SelectedRegionIndex::pairs_into(std::uint32_t tile) const noexcept {
+  const std::size_t a = in_pair_offsets_[tile], b = in_pair_offsets_[tile + 1U];
+  return {std::span<const std::uint32_t>(in_pair_tiles_).subspan(a, b - a),
+          std::span<const std::uint64_t>(in_pair_edges_).subspan(a, b - a)};
 }
 
 std::uint64_t SelectedRegionIndex::selected_vertex_count(std::span<const TileId> tiles) const {

@@ -206,6 +206,21 @@ DeviceGraphFingerprint fingerprint_device_graph_layout32(const DeviceGraphLayout
   return f.finish();
 }
 
+DeviceGraphFingerprint fingerprint_device_graph_source32(const WeightedGraph& graph,
+                                                         const TileRunLayout64& tile_runs) {
+  // The tile runs are a function of the graph, so the graph's memo holds the
+  // value once runs equal to the graph's own have been seen.
+  if (tile_runs != build_tile_run_layout(graph))
+    throw std::invalid_argument{"device graph fingerprint: tile runs are not the graph's"};
+  WeightedGraphMemo& memo = graph.memo();
+  std::call_once(memo.device_fingerprint_once, [&] {
+    const DeviceGraphFingerprint f = fingerprint_device_graph_layout32(build_device_graph_layout32(graph, tile_runs));
+    memo.device_fingerprint[0] = f.first;
+    memo.device_fingerprint[1] = f.second;
+  });
+  return DeviceGraphFingerprint{memo.device_fingerprint[0], memo.device_fingerprint[1]};
+}
+
 void compute_tile_run_lane_masks(const WeightedGraph& graph, const TileRunLayout64& runs,
                                  std::span<const std::uint32_t> tile_lane_masks,
                                  TileRunLaneMasks& output) {

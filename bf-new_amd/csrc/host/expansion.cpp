@@ -459,10 +459,9 @@ bool directory_matches(const TileDirectory& a, const TileDirectory& b) {
          std::ranges::equal(a.halo_vertices(), b.halo_vertices());
 }
 
-std::uint64_t comparison_fingerprint(const WeightedGraph& graph, const TileRunLayout64& runs,
-                                     std::span<const QueryState> states, const BatchedExpansionOptions& o) {
+std::uint64_t comparison_fingerprint(const DeviceGraphFingerprint& g, std::span<const QueryState> states,
+                                     const BatchedExpansionOptions& o) {
   Fnv f;
-  const DeviceGraphFingerprint g = fingerprint_device_graph_layout32(build_device_graph_layout32(graph, runs));
   f.word(g.first);
   f.word(g.second);
   const GpuRunOptions& r = o.run_options;
@@ -575,8 +574,9 @@ BatchedExpansionRunResult run_batched_expansion(const WeightedGraph& graph, cons
     throw std::invalid_argument{std::string{"batched expansion: invalid graph: "} + why};
   if (!directory_matches(directory, build_tile_directory(graph)))
     throw std::invalid_argument{"batched expansion: the tile directory is not the graph's"};
-  if (tile_runs != build_tile_run_layout(graph))
-    throw std::invalid_argument{"batched expansion: the tile runs are not the graph's"};
+  // Also checks that the tile runs are the graph's (std::invalid_argument
+  // otherwise); the graph keeps the value, so later runs do not pay again.
+  const DeviceGraphFingerprint graph_identity = fingerprint_device_graph_source32(graph, tile_runs);
 
   const auto t_total = Clock::now();
   BatchedExpansionRunResult out;
@@ -606,7 +606,7 @@ BatchedExpansionRunResult run_batched_expansion(const WeightedGraph& graph, cons
   for (std::size_t i = 1; i < states.size(); ++i)
     if (states[i - 1].query.query_id == states[i].query.query_id)
       throw std::invalid_argument{"batched expansion: query ids must be unique"};
-  m.schedule_comparison_fingerprint = comparison_fingerprint(graph, tile_runs, states, options);
+  m.schedule_comparison_fingerprint = comparison_fingerprint(graph_identity, states, options);
 
   const auto state_of = [&states](QueryId id) -> QueryState& {
     const auto it = std::lower_bound(states.begin(), states.end(), id,

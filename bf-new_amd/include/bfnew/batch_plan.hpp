@@ -51,12 +51,26 @@ class SelectedRegionIndex {
   [[nodiscard]] std::uint64_t selected_vertex_count(std::span<const TileId> tiles) const;
   [[nodiscard]] std::uint64_t selected_edge_count(std::span<const TileId> tiles) const;
 
+  // Per-tile counts for incremental region arithmetic (tile < tile_count()).
+  struct TilePairs {
+    std::span<const std::uint32_t> tiles;  // the other tile, ascending
+    std::span<const std::uint64_t> edges;  // edges between the two
+  };
+  [[nodiscard]] std::uint64_t tile_vertex_count(std::uint32_t tile) const noexcept {
+    return vertex_counts_[tile];
+  }
+  [[nodiscard]] TilePairs pairs_from(std::uint32_t tile) const noexcept;
+  [[nodiscard]] TilePairs pairs_into(std::uint32_t tile) const noexcept;
+
  private:
   const WeightedGraph* graph_;
   std::vector<std::uint64_t> vertex_counts_;
   std::vector<std::uint64_t> pair_offsets_;   // [tile_count + 1]
   std::vector<std::uint32_t> pair_tiles_;     // destination tile, ascending per source tile
   std::vector<std::uint64_t> pair_edges_;     // edges from the source tile into it
+  std::vector<std::uint64_t> in_pair_offsets_;  // the same pairs grouped by destination tile
+  std::vector<std::uint32_t> in_pair_tiles_;    // source tile, ascending per destination tile
+  std::vector<std::uint64_t> in_pair_edges_;
 };
 
 inline constexpr std::uint32_t invalid_batch_query_index = 0xFFFFFFFFu;

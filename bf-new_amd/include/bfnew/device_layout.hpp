@@ -81,6 +81,13 @@ struct DeviceGraphFingerprint {
 [[nodiscard]] DeviceGraphFingerprint fingerprint_device_graph_layout32(
     const DeviceGraphLayout32& layout) noexcept;
 
+// fingerprint_device_graph_layout32 of the layout built from the graph and
+// its tile runs (reference: src/device_layout.cpp:711-760).  Throws
+// std::invalid_argument when tile_runs are not the graph's.  The value is
+// kept with the graph (WeightedGraph::memo), so a graph pays for it once.
+[[nodiscard]] DeviceGraphFingerprint fingerprint_device_graph_source32(const WeightedGraph& graph,
+                                                                       const TileRunLayout64& tile_runs);
+
 struct TileRunLaneMasks {
   std::vector<std::uint32_t> csr_run_masks;
   std::vector<std::uint32_t> csc_run_masks;

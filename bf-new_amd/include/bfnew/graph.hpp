@@ -7,6 +7,8 @@
 #include <cstddef>
 #include <cstdint>
 #include <limits>
+#include <memory>
+#include <mutex>
 #include <span>
 #include <vector>
 
@@ -74,6 +76,13 @@ class InputGraph {
 struct SyntheticTileGridConfig;
 class SyntheticTileGraph;
 
+// Values derived from an immutable WeightedGraph, computed at most once per
+// graph and shared by its copies (which hold the same data).
+struct WeightedGraphMemo {
+  std::once_flag device_fingerprint_once;
+  std::uint64_t device_fingerprint[2]{};
+};
+
 class WeightedGraph {
  public:
   [[nodiscard]] std::uint32_t vertex_count() const noexcept { return vertex_count_; }
@@ -96,6 +105,7 @@ class WeightedGraph {
   [[nodiscard]] std::span<const TileCoordinate> tile_coordinates() const noexcept { return tile_coordinates_; }
   [[nodiscard]] std::span<const EdgeOffset> tile_vertex_offsets() const noexcept { return tile_vertex_offsets_; }
   [[nodiscard]] const SpatialOrderConfig& spatial_order_config() const noexcept { return config_; }
+  [[nodiscard]] WeightedGraphMemo& memo() const noexcept { return *memo_; }
 
  private:
   WeightedGraph() = default;
@@ -118,6 +128,7 @@ class WeightedGraph {
   std::vector<TileId> owner_tiles_;
   std::vector<TileCoordinate> tile_coordinates_;
   std::vector<EdgeOffset> tile_vertex_offsets_;
+  std::shared_ptr<WeightedGraphMemo> memo_ = std::make_shared<WeightedGraphMemo>();
 
   friend WeightedGraph build_weighted_graph(const InputGraph& input);
   friend WeightedGraph build_spatially_ordered_graph(const InputGraph& input,

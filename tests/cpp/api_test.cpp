@@ -478,6 +478,110 @@ BatchPlannerPolicy planner_policy(std::uint32_t width) {
 
 ResourceClassId rc_of_test() { return ResourceClassId{1U}; }
 
+// The greedy selection rule written out directly: every open batch scores
+// every remaining query from scratch (Jaccard, shared terminal tiles,
+// inflation, union edges / vertices / new tiles, gaps to the anchor, id) and
+// admits the best until the width or the thresholds stop it.  The library
+// computes the same plan incrementally; the two must agree exactly.
+BatchPlan exhaustive_plan(const SelectedRegionIndex& index, const std::vector<BatchQueryFeatures>& qs,
+                          const BatchPlannerPolicy& policy) {
+  const auto frac_less = [](BatchFraction a, BatchFraction b) {
+    return static_cast<unsigned __int128>(a.numerator) * b.denominator <
+           static_cast<unsigned __int128>(b.numerator) * a.denominator;
+  };
+  const auto unite = [](const std::vector<TileId>& a, const std::vector<TileId>& b) {
+    std::vector<TileId> out;
+    std::set_union(a.begin(), a.end(), b.begin(), b.end(), std::back_inserter(out));
+    return out;
+  };
+  const auto meet = [](const std::vector<TileId>& a, const std::vector<TileId>& b) {
+    std::vector<TileId> out;
+    std::set_intersection(a.begin(), a.end(), b.begin(), b.end(), std::back_inserter(out));
+    return out.size();
+  };
+  const auto dist = [](std::uint32_t a, std::uint32_t b) { return a > b ? a - b : b - a; };
+  BatchPlan plan;
+  plan.policy = policy;
+  plan.input_query_count = static_cast<std::uint32_t>(qs.size());
+  std::vector<std::uint32_t> order(qs.size());
+  for (std::uint32_t i = 0; i < order.size(); ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](std::uint32_t a, std::uint32_t b) {
+    return std::make_tuple(qs[b].selected_edge_estimate, qs[b].selected_vertex_count, qs[b].selected_tiles.size(),
+                           qs[a].expansion_generation, qs[a].query_id) <
+           std::make_tuple(qs[a].selected_edge_estimate, qs[a].selected_vertex_count, qs[a].selected_tiles.size(),
+                           qs[b].expansion_generation, qs[b].query_id);
+  });
+  const BatchFraction jmin{policy.minimum_jaccard_numerator, policy.minimum_jaccard_denominator};
+  const BatchFraction imax{policy.maximum_union_inflation_numerator, policy.maximum_union_inflation_denominator};
+  std::vector<bool> taken(qs.size(), false);
+  for (std::uint32_t a : order) {
+    if (taken[a]) continue;
+    taken[a] = true;
+    std::vector<std::uint32_t> lanes{a};
+    std::vector<TileId> tiles = qs[a].selected_tiles, src = qs[a].source_tiles, tgt = qs[a].target_tiles;
+    std::uint64_t lane_tiles = tiles.size();
+    while (lanes.size() < policy.lane_width) {
+      using Key = std::tuple<std::uint64_t, std::uint64_t, std::uint64_t, std::uint32_t, std::uint32_t,
+                             std::uint32_t, std::uint32_t>;
+      bool found = false;
+      std::uint32_t best = 0;
+      BatchFraction bj{}, bi{};
+      std::uint64_t bs = 0;
+      Key bk{};
+      for (std::uint32_t q = 0; q < qs.size(); ++q) {
+        if (taken[q]) continue;
+        const auto u = unite(tiles, qs[q].selected_tiles);
+        const BatchFraction j{meet(tiles, qs[q].selected_tiles), u.size()};
+        const BatchFraction inf{(lanes.size() + 1U) * u.size(), lane_tiles + qs[q].selected_tiles.size()};
+        if (frac_less(j, jmin) || frac_less(imax, inf)) continue;
+        const std::uint64_t shared = meet(src, qs[q].source_tiles) + meet(tgt, qs[q].target_tiles);
+        const Key k{index.selected_edge_count(u), index.selected_vertex_count(u), u.size() - tiles.size(),
+                    dist(qs[q].expansion_generation, qs[a].expansion_generation),
+                    dist(qs[q].source_count, qs[a].source_count), dist(qs[q].target_count, qs[a].target_count),
+                    qs[q].query_id.value()};
+        bool better = !found;
+        if (!better) {
+          if (frac_less(bj, j) || frac_less(j, bj)) better = frac_less(bj, j);
+          else if (shared != bs) better = shared > bs;
+          else if (frac_less(inf, bi) || frac_less(bi, inf)) better = frac_less(inf, bi);
+          else better = k < bk;
+        }
+        if (better) found = true, best = q, bj = j, bi = inf, bs = shared, bk = k;
+      }
+      if (!found) break;
+      taken[best] = true;
+      lanes.push_back(best);
+      lane_tiles += qs[best].selected_tiles.size();
+      tiles = unite(tiles, qs[best].selected_tiles);
+      src = unite(src, qs[best].source_tiles);
+      tgt = unite(tgt, qs[best].target_tiles);
+    }
+    BatchPlanEntry e;
+    e.lane_width = policy.lane_width;
+    e.valid_lane_mask = lanes.size() >= 32 ? ~LaneMask{0} : (LaneMask{1} << lanes.size()) - 1U;
+    e.query_indices_by_lane.assign(policy.lane_width, invalid_batch_query_index);
+    e.query_ids_by_lane.assign(policy.lane_width, invalid_batch_query_id);
+    e.expansion_generations_by_lane.assign(policy.lane_width, 0U);
+    e.union_tiles = tiles;
+    e.union_tile_lane_masks.assign(tiles.size(), 0U);
+    for (std::uint32_t l = 0; l < lanes.size(); ++l) {
+      const BatchQueryFeatures& f = qs[lanes[l]];
+      e.query_indices_by_lane[l] = lanes[l];
+      e.query_ids_by_lane[l] = f.query_id;
+      e.expansion_generations_by_lane[l] = f.expansion_generation;
+      e.selected_lane_vertex_count += f.selected_vertex_count;
+      e.selected_lane_edge_estimate += f.selected_edge_estimate;
+      for (std::size_t t = 0; t < tiles.size(); ++t)
+        if (std::binary_search(f.selected_tiles.begin(), f.selected_tiles.end(), tiles[t]))
+          e.union_tile_lane_masks[t] |= LaneMask{1} << l;
+    }
+    e.union_vertex_count = index.selected_vertex_count(tiles);
+    e.union_edge_estimate = index.selected_edge_count(tiles);
+    plan.batches.push_back(std::move(e));
+  }
+  return plan;
+}
+
 void planner(const char* golden_path) {
   const PlannerFixture fx = planner_fixture();
   const WeightedGraph& g = fx.partitioned.graph;
@@ -678,6 +782,50 @@ void planner(const char* golden_path) {
       CHECK(uni == e.union_tiles);
     }
     CHECK(std::all_of(seen.begin(), seen.end(), [](int n) { return n == 1; }));
+    CHECK(rp == exhaustive_plan(rindex, rf, pol));
+  }
+  CHECK(make_overlapping_batch_plan(index, features, planner_policy(8U)) ==
+        exhaustive_plan(index, features, planner_policy(8U)));
+
+  // Overlapping boxes on a 24x24x16 synthetic grid (256 queries, generations
+  // 0-2, boxes grown by 0-3 rings): the incremental planner equals the
+  // written-out rule at every width and under loose and strict thresholds.
+  SyntheticTileGridConfig sc;
+  sc.tiles_x = 24;
+  sc.tiles_y = 24;
+  sc.nodes_per_tile = 16;
+  sc.seed = 9;
+  const SyntheticTileGraph sg = SyntheticTileGraph::generate(sc, 1);
+  const WeightedGraph sgraph = sg.weighted_graph();
+  const TileRunLayout64 sruns = build_tile_run_layout(sgraph);
+  const SelectedRegionIndex sindex{sgraph, sruns};
+  SyntheticQueryConfig sqc;
+  sqc.count = 256;
+  sqc.seed = 13;
+  sqc.side_min = 2;
+  sqc.side_max = 12;
+  std::vector<RouteQuery> sq;
+  std::uint32_t k = 0;
+  for (const SyntheticQuery& q : generate_synthetic_queries(sc, sqc)) {
+    const VertexId s1[1] = {VertexId{q.source}}, t1[1] = {VertexId{q.target}};
+    sq.push_back(make_route_query(QueryId{q.query_id}, sgraph, s1, t1, k % 4U, k % 3U));
+    ++k;
+  }
+  const auto sf = make_batch_query_features(sgraph, sindex, sq);
+  for (std::uint32_t w : {8U, 16U, 32U}) {
+    BatchPlannerPolicy loose;
+    loose.lane_width = w;
+    loose.minimum_jaccard_numerator = 0U;
+    loose.maximum_union_inflation_numerator = 16U;
+    BatchPlannerPolicy strict = loose;
+    strict.minimum_jaccard_numerator = 1U;
+    strict.minimum_jaccard_denominator = 4U;
+    strict.maximum_union_inflation_numerator = 2U;
+    for (const BatchPlannerPolicy& pol : {loose, strict, planner_policy(w)}) {
+      const BatchPlan fast = make_overlapping_batch_plan(sindex, sf, pol);
+      CHECK(fast == exhaustive_plan(sindex, sf, pol));
+      CHECK(validate_batch_plan(sindex, sf, fast).ok());
+    }
   }
 }
 
