@@ -328,22 +328,25 @@ EXPANSION_DRIVER = os.path.join(ROOT, "bf-new_amd", "lib", "bfnew_expansion_benc
 
 
 def expansion(tune):
-    """run_batched_expansion over the device executor (compact paths) on the
-    64x64x256 graph, 256 queries starting from their two terminal tiles; one
-    record per expansion schedule (bf-new_amd/csrc/bench/expansion_bench.cpp)."""
+    """run_batched_expansion (compact paths) on the 64x64x256 graph, 256
+    queries starting from their two terminal tiles, one record per expansion
+    schedule, for both device runners: one run per planned lane batch and one
+    run per planning pass (bf-new_amd/csrc/bench/expansion_bench.cpp)."""
     import subprocess
-    cmd = [EXPANSION_DRIVER, "--tiles", "64", "64", "256", "--queries", "256",
-           "--engine", str(tune["engine"]), "--control", str(bfn.CHUNKED),
-           "--sweeps", str(tune["sweeps"]), "--boundary", str(tune["boundary_mode"]),
-           "--graph", str(tune["use_graph"]), "--graph-rounds", "8",
-           "--claim", str(tune["items_per_claim"]), "--min-waves", str(tune["min_waves"])]
-    log("expansion:", " ".join(cmd[1:]))
-    p = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
-    sys.stderr.write(p.stderr)
-    if p.returncode != 0:
-        raise SystemExit(f"expansion driver failed ({p.returncode})")
-    out = json.loads(p.stdout.strip().splitlines()[-1])
-    log("expansion:", json.dumps(out))
+    out = {}
+    for runner in ("batch", "pass"):
+        cmd = [EXPANSION_DRIVER, "--tiles", "64", "64", "256", "--queries", "256", "--runner", runner,
+               "--engine", str(tune["engine"]), "--control", str(bfn.CHUNKED),
+               "--sweeps", str(tune["sweeps"]), "--boundary", str(tune["boundary_mode"]),
+               "--graph", str(tune["use_graph"]), "--graph-rounds", "8",
+               "--claim", str(tune["items_per_claim"]), "--min-waves", str(tune["min_waves"])]
+        log("expansion:", " ".join(cmd[1:]))
+        p = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        sys.stderr.write(p.stderr)
+        if p.returncode != 0:
+            raise SystemExit(f"expansion driver failed ({p.returncode})")
+        out[runner] = json.loads(p.stdout.strip().splitlines()[-1])
+        log("expansion:", json.dumps(out[runner]))
     return out
 
 

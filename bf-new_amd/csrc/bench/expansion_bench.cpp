@@ -14,7 +14,11 @@
 //                         [--sweeps S] [--boundary B] [--graph G]
 //                         [--graph-rounds R] [--claim K] [--min-waves M]
 //                         [--mode compact_paths|compact_status] [--reps R]
-//                         [--start terminals|rectangle]
+//                         [--start terminals|rectangle] [--runner batch|pass]
+//
+// --runner batch: hip::BatchedExpansionExecutor, one device run per planned
+// lane batch; --runner pass: hip::PassBatchedExpansionExecutor, one device run
+// per planning pass.
 #include "bfnew/batched_expansion.hpp"
 #include "bfnew/device_layout.hpp"
 #include "bfnew/hip/batched_expansion.hpp"
@@ -43,6 +47,7 @@ struct Args {
                 claim = 2, min_waves = 0;
   bool paths = true;
   bool terminal_tiles_only = true;
+  bool per_pass = false;
 };
 
 Args parse(int argc, char** argv) {
@@ -73,6 +78,9 @@ Args parse(int argc, char** argv) {
     else if (k == "--mode") {
       if (i + 1 >= argc) throw std::invalid_argument{"missing value for --mode"};
       a.paths = std::strcmp(argv[++i], "compact_status") != 0;
+    } else if (k == "--runner") {
+      if (i + 1 >= argc) throw std::invalid_argument{"missing value for --runner"};
+      a.per_pass = std::strcmp(argv[++i], "pass") == 0;
     } else if (k == "--start") {
       if (i + 1 >= argc) throw std::invalid_argument{"missing value for --start"};
       a.terminal_tiles_only = std::strcmp(argv[++i], "rectangle") != 0;
@@ -150,9 +158,10 @@ int main(int argc, char** argv) {
     o.maximum_expansions = 4U;
     o.terminal_policy = ExpansionTerminalPolicy::full_region_fallback;
     o.enable_compact_paths = a.paths ? 1U : 0U;
-    hip::BatchedExpansionExecutor exec(g, runs, resident, ws, stream, o.run_options,
-                                       a.paths ? hip::BatchedExpansionTransferMode::compact_paths
-                                               : hip::BatchedExpansionTransferMode::compact_status);
+    const auto mode = a.paths ? hip::BatchedExpansionTransferMode::compact_paths
+                              : hip::BatchedExpansionTransferMode::compact_status;
+    hip::BatchedExpansionExecutor batch_exec(g, runs, resident, ws, stream, o.run_options, mode);
+    hip::PassBatchedExpansionExecutor pass_exec(g, runs, resident, ws, stream, o.run_options, mode);
     bfn_tuning tune;
     bfn_tuning_default(&tune);
     tune.tile_sweeps = a.sweeps;
@@ -161,7 +170,10 @@ int main(int argc, char** argv) {
     tune.graph_rounds = a.graph_rounds;
     tune.items_per_claim = a.claim;
     tune.min_waves = a.min_waves;
-    exec.set_tuning(tune);
+    batch_exec.set_tuning(tune);
+    pass_exec.set_tuning(tune);
+    const ExpansionBatchRunner runner =
+        a.per_pass ? ExpansionBatchRunner{std::ref(pass_exec)} : ExpansionBatchRunner{std::ref(batch_exec)};
 
     const std::array<ExpansionSchedulePolicy, 4> schedules{one_ring_expansion(), fixed_ring_expansion(2U),
                                                            doubling_margin_expansion(),
@@ -170,14 +182,14 @@ int main(int argc, char** argv) {
     std::string rows;
     // One untimed run first: device allocations and code objects warm up.
     o.schedule = schedules[0];
-    (void)run_batched_expansion(g, dir, runs, queries, o, std::ref(exec));
+    (void)run_batched_expansion(g, dir, runs, queries, o, runner);
     for (std::size_t i = 0; i < schedules.size(); ++i) {
       o.schedule = schedules[i];
       BatchedExpansionRunResult best;
       double best_ms = 0.0;
       for (std::uint32_t r = 0; r < std::max(1U, a.reps); ++r) {
         const auto t0 = std::chrono::steady_clock::now();
-        BatchedExpansionRunResult run = run_batched_expansion(g, dir, runs, queries, o, std::ref(exec));
+        BatchedExpansionRunResult run = run_batched_expansion(g, dir, runs, queries, o, runner);
         const double ms = ms_since(t0);
         if (r == 0 || ms < best_ms) {
           best_ms = ms;
@@ -221,10 +233,11 @@ int main(int argc, char** argv) {
     }
     const auto chosen = select_expansion_schedule_from_evidence(evidence);
     std::printf(
-        "{\"workload\": \"expansion_%ux%ux%u_q%u\", \"start\": \"%s\", \"lane_width\": %u, \"engine\": %u, "
+        "{\"workload\": \"expansion_%ux%ux%u_q%u\", \"runner\": \"%s\", \"start\": \"%s\", \"lane_width\": %u, \"engine\": %u, "
         "\"control\": %u, \"mode\": \"%s\", \"maximum_expansions\": 4, \"terminal_policy\": "
         "\"full_region_fallback\", \"host_build_ms\": %.1f, \"selected_schedule\": \"%s\", \"schedules\": [%s]}\n",
-        a.tx, a.ty, a.npt, a.queries, a.terminal_tiles_only ? "terminal_tiles" : "terminal_rectangle", a.lane_width, a.engine, a.control, a.paths ? "compact_paths" : "compact_status",
+        a.tx, a.ty, a.npt, a.queries, a.per_pass ? "one device run per planning pass" : "one device run per lane batch",
+        a.terminal_tiles_only ? "terminal_tiles" : "terminal_rectangle", a.lane_width, a.engine, a.control, a.paths ? "compact_paths" : "compact_status",
         build_ms, chosen ? schedule_name(chosen->kind) : "none (tie or incomplete evidence)", rows.c_str());
     return 0;
   } catch (const std::exception& e) {

@@ -572,7 +572,7 @@ BatchedExpansionRunResult run_batched_expansion(const WeightedGraph& graph, cons
   if (!graph.has_spatial_ordering()) throw std::invalid_argument{"batched expansion: graph is not spatially ordered"};
   if (const char* why = validate_weighted_graph_message(graph); why && !std::string_view{why}.empty())
     throw std::invalid_argument{std::string{"batched expansion: invalid graph: "} + why};
-  if (!directory_matches(directory, build_tile_directory(graph)))
+  if (!directory.built_from(graph) && !directory_matches(directory, build_tile_directory(graph)))
     throw std::invalid_argument{"batched expansion: the tile directory is not the graph's"};
   // Also checks that the tile runs are the graph's (std::invalid_argument
   // otherwise); the graph keeps the value, so later runs do not pay again.

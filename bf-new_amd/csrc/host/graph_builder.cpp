@@ -320,6 +320,7 @@ TileDirectory build_tile_directory(const WeightedGraph& g) {
     for (auto p = in.column_offsets[d]; p < in.column_offsets[d + 1]; ++p)
       if (owner[in.sources[p].value()] != owner[d]) incoming[owner[d].value()].push_back(in.edge_ids[p]);
   TileDirectory dir;
+  dir.built_from_ = g.memo_handle();
   dir.spill_tile_ = spill;
   auto flatten = [](auto& groups, auto& offsets, auto& values) {
     offsets.assign(1, 0);
@@ -358,6 +359,11 @@ PartitionedGraph UniformGridPartitioner::partition(const InputGraph& input) cons
   WeightedGraph graph = build_spatially_ordered_graph(input, config_, locality_policy_);
   TileDirectory directory = build_tile_directory(graph);
   return PartitionedGraph{std::move(graph), std::move(directory)};
+}
+
+bool TileDirectory::built_from(const WeightedGraph& graph) const noexcept {
+  const auto memo = built_from_.lock();
+  return memo && memo.get() == &graph.memo();
 }
 
 }  // namespace bfnew

@@ -106,6 +106,7 @@ class WeightedGraph {
   [[nodiscard]] std::span<const EdgeOffset> tile_vertex_offsets() const noexcept { return tile_vertex_offsets_; }
   [[nodiscard]] const SpatialOrderConfig& spatial_order_config() const noexcept { return config_; }
   [[nodiscard]] WeightedGraphMemo& memo() const noexcept { return *memo_; }
+  [[nodiscard]] std::weak_ptr<const WeightedGraphMemo> memo_handle() const noexcept { return memo_; }
 
  private:
   WeightedGraph() = default;

@@ -61,4 +61,48 @@ class BatchedExpansionExecutor final {
   std::unique_ptr<Impl> impl_;
 };
 
+// The same runner contract with one device run per planning pass: the first
+// call of a pass (batch_index 0, or a pass whose queries differ from the
+// last run's) submits every query of the pass as one batch -- the engines
+// keep labels per (query, selected tile), so a batch needs no lanes and a
+// query's labels, rounds and classification do not depend on the others --
+// and every planned lane batch of the pass is answered from that run.
+// Outcomes, classifications and compact payloads equal
+// BatchedExpansionExecutor's; the device sees a few large launches that fill
+// the chip instead of one small launch per lane batch.  Transfer modes
+// compact_status and compact_paths; the pass's host timing and its
+// path-offset words are reported with the batch that ran it.
+class PassBatchedExpansionExecutor final {
+ public:
+  // Throws std::invalid_argument for invalid options, a transfer mode other
+  // than compact_status / compact_paths, or compact paths with
+  // instrumentation.
+  PassBatchedExpansionExecutor(const WeightedGraph& host_graph, const TileRunLayout64& tile_runs,
+                               const ResidentDeviceGraph& resident_graph,
+                               ReusableBatchedWorkspace& workspace, const HipStream& stream,
+                               GpuRunOptions run_options,
+                               BatchedExpansionTransferMode transfer_mode =
+                                   BatchedExpansionTransferMode::compact_status);
+  ~PassBatchedExpansionExecutor();
+  PassBatchedExpansionExecutor(const PassBatchedExpansionExecutor&) = delete;
+  PassBatchedExpansionExecutor& operator=(const PassBatchedExpansionExecutor&) = delete;
+  PassBatchedExpansionExecutor(PassBatchedExpansionExecutor&&) = delete;
+  PassBatchedExpansionExecutor& operator=(PassBatchedExpansionExecutor&&) = delete;
+
+  [[nodiscard]] EngineKind kind() const noexcept;
+  [[nodiscard]] BatchedExpansionTransferMode transfer_mode() const noexcept;
+  void set_tuning(const bfn_tuning& tuning) noexcept;
+  // Device runs made so far (one per planning pass served).
+  [[nodiscard]] std::uint64_t device_runs() const noexcept;
+
+  [[nodiscard]] ExpansionBatchExecution operator()(std::span<const RouteQuery> queries,
+                                                   std::span<const BatchQueryFeatures> features,
+                                                   const BatchPlanEntry& batch,
+                                                   const ExpansionBatchContext& context);
+
+ private:
+  class Impl;
+  std::unique_ptr<Impl> impl_;
+};
+
 }  // namespace bfnew::hip

@@ -7,6 +7,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <functional>
+#include <memory>
 #include <span>
 #include <vector>
 
@@ -15,6 +16,7 @@ namespace bfnew {
 class InputGraph;
 class WeightedGraph;
 struct PartitionedGraph;
+struct WeightedGraphMemo;
 
 struct SpatialOrderConfig {
   std::int32_t origin_x{};
@@ -65,8 +67,12 @@ class TileDirectory {
   [[nodiscard]] std::span<const EdgeId> incoming_cross_edge_ids() const noexcept { return incoming_cross_edge_ids_; }
   [[nodiscard]] std::span<const EdgeOffset> halo_vertex_offsets() const noexcept { return halo_vertex_offsets_; }
   [[nodiscard]] std::span<const VertexId> halo_vertices() const noexcept { return halo_vertices_; }
+  // True when this directory was built from `graph` or from a copy of it
+  // (graphs are immutable; copies share one memo).
+  [[nodiscard]] bool built_from(const WeightedGraph& graph) const noexcept;
 
  private:
+  std::weak_ptr<const WeightedGraphMemo> built_from_;
   TileId spill_tile_{};
   std::vector<EdgeOffset> neighbor_tile_offsets_;
   std::vector<TileId> neighbor_tiles_;

@@ -896,6 +896,13 @@ void host_mode() {
     CHECK(std::binary_search(oc.final_query.selected_tiles.begin(), oc.final_query.selected_tiles.end(),
                              dir.spill_tile()));
     CHECK(oc.final_distances.size() == sg.vertex_count() && same_bits(oc.final_distances[t[0].value()], 3.0F));
+    // A directory built from another graph is rejected.
+    CHECK(dir.built_from(sg) && !dir.built_from(g));
+    const BatchedExpansionOptions fo = options_of(EngineKind::jacobi_pull, one_ring_expansion());
+    HostRunner fixture_runner{g, fo.run_options};
+    CHECK(throws<std::invalid_argument>([&] {
+      (void)run_batched_expansion(g, dir, f.tile_runs, f.subset({kImmediate}), fo, std::ref(fixture_runner));
+    }));
   }
 }
 
@@ -946,6 +953,22 @@ int gpu_mode() {
                   tag);
       CHECK_MSG(got.metrics.schedule_comparison_fingerprint == want.metrics.schedule_comparison_fingerprint, tag);
 
+      // One device run per planning pass: the same outcomes and traces.
+      hip::PassBatchedExpansionExecutor pass_exec(g, f.tile_runs, resident, ws, stream, run);
+      const auto by_pass = run_batched_expansion(g, f.directory, f.tile_runs, queries, o, std::ref(pass_exec));
+      CHECK_MSG(pass_exec.device_runs() == by_pass.metrics.planning_passes, tag);
+      CHECK_MSG(by_pass.trace.size() == want.trace.size(), tag);
+      for (std::size_t i = 0; i < std::min(by_pass.trace.size(), want.trace.size()); ++i)
+        CHECK_MSG(by_pass.trace[i].reached_lane_mask == want.trace[i].reached_lane_mask &&
+                      by_pass.trace[i].miss_lane_mask == want.trace[i].miss_lane_mask &&
+                      by_pass.trace[i].query_ids_by_lane == want.trace[i].query_ids_by_lane,
+                  tag + " pass runner");
+      for (std::size_t i = 0; i < want.queries.size(); ++i)
+        CHECK_MSG(by_pass.queries[i].final_query == want.queries[i].final_query &&
+                      by_pass.queries[i].disposition == want.queries[i].disposition &&
+                      by_pass.queries[i].attempts == want.queries[i].attempts,
+                  tag + " pass runner");
+
       // Compact paths: payloads equal the host runner's (the fixture has no
       // ties, so the tight path is unique).
       o.enable_compact_paths = 1U;
@@ -959,6 +982,16 @@ int gpu_mode() {
       CHECK_MSG(got_p.metrics.compact_total_device_to_host_bytes >
                     got_p.metrics.compact_transfer.total_bytes,
                 tag);
+      hip::PassBatchedExpansionExecutor pass_paths(g, f.tile_runs, resident, ws, stream, run,
+                                                   hip::BatchedExpansionTransferMode::compact_paths);
+      const auto pass_p = run_batched_expansion(g, f.directory, f.tile_runs, queries, o, std::ref(pass_paths));
+      CHECK_MSG(pass_p.metrics.compact_transfer == want_p.metrics.compact_transfer, tag + " pass runner");
+      for (std::size_t i = 0; i < want_p.queries.size(); ++i) {
+        const ExpansionQueryOutcome &x = pass_p.queries[i], &y = want_p.queries[i];
+        CHECK_MSG(x.final_query == y.final_query && x.disposition == y.disposition && x.attempts == y.attempts &&
+                      x.compact_paths == y.compact_paths,
+                  tag + " pass runner paths of query " + std::to_string(y.final_query.query_id.value()));
+      }
       for (std::size_t i = 0; i < want_p.queries.size(); ++i) {
         const ExpansionQueryOutcome &x = got_p.queries[i], &y = want_p.queries[i];
         CHECK_MSG(x.final_query == y.final_query && x.disposition == y.disposition && x.attempts == y.attempts,

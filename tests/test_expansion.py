@@ -54,7 +54,8 @@ def test_expansion_orchestrator_executor():
     _run("gpu")
 
 
-def test_expansion_bench_driver_emulated():
+@pytest.mark.parametrize("runner", ["batch", "pass"])
+def test_expansion_bench_driver_emulated(runner):
     """The bench driver (bench.py --expansion) end to end on the CPU emulation,
     at a size the emulation finishes in seconds: every query is reached, the
     misses of the terminal-tiles start expand once, and the records add up."""
@@ -68,7 +69,8 @@ def test_expansion_bench_driver_emulated():
     env.update(EMU_CUS="2", EMU_OCC="2", EMU_YIELD_PCT="0")
     driver = os.path.join(ROOT, "bf-new_amd", "lib", "bfnew_expansion_bench")
     p = subprocess.run([driver, "--tiles", "8", "8", "16", "--queries", "24", "--lane-width", "8",
-                        "--reps", "1"], capture_output=True, text=True, timeout=900, env=env)
+                        "--reps", "1", "--runner", runner], capture_output=True, text=True, timeout=900,
+                       env=env)
     assert p.returncode == 0, p.stdout + p.stderr
     out = json.loads(p.stdout.strip().splitlines()[-1])
     assert out["workload"] == "expansion_8x8x16_q24" and len(out["schedules"]) == 4
