@@ -15,9 +15,14 @@ is kept only when it beats the best uniform engine on the whole batch.
 
 Output: tuning/tuning_table.json (one entry per workload name, read by
 bench.py) and a TSV record of every trial (tuning/trials_<workload>.tsv), so a
-decision can be audited and replayed.  The reference keeps the same kind of
-evidence as shootout manifests (src/engine_shootout.cpp:787-1567, TSV at
-:2189).
+decision can be audited and replayed.
+
+--shootout runs the per-query campaign of the reference's engine shootout on
+the workload instead: every query alone under each configuration of a
+catalog (the three engines, per-round host poll), one warmup and --reps timed
+repetitions, and writes the reference's manifest, catalog and samples TSV
+files (shootout.py) under tuning/shootout_<workload>/, with the
+configuration of smallest median wall time.
 """
 from __future__ import annotations
 
@@ -35,6 +40,7 @@ sys.path.insert(0, HERE)
 import numpy as np  # noqa: E402
 
 import bfnew_amd as bfn  # noqa: E402
+import shootout  # noqa: E402
 
 WORKLOADS = {
     "c3_batched_128x128x512_q256": dict(tiles=(128, 128, 512), seed=3, queries=256, query_seed=33,
@@ -102,6 +108,116 @@ def candidates(engines, blocks, quick):
     return out
 
 
+def _fnv_words(values):
+    """Two 64-bit FNV-1a lanes (offset bases 14695981039346656037 and its
+    byte-reversal) over the little-endian u64 encoding of `values`."""
+    a, b = 14695981039346656037, 0x25233184D3AB5CCB
+    for v in values:
+        for k in range(8):
+            byte = (int(v) >> (8 * k)) & 0xFF
+            a = ((a ^ byte) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+            b = ((b ^ byte) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return a, b
+
+
+def shootout_campaign(args, cfg, dev, hg, dg, qs, tiles):
+    """Every query alone under each catalog configuration: a point-Jacobi
+    pilot for the expected round count, then one warmup and args.reps timed
+    reruns per (configuration, query); returns the manifest, catalog and
+    samples."""
+    arrays = hg.layout_arrays()
+    owner = arrays["owner_tiles"]
+    tile_vertices = np.diff(arrays["tile_vertex_offsets"].astype(np.int64))
+    src = np.repeat(np.arange(arrays["vertex_count"], dtype=np.int64),
+                    np.diff(arrays["csr_row_offsets"].astype(np.int64)))
+    src_tile, dst_tile = owner[src], owner[arrays["csr_destinations"]]
+    graph_words = hg.fingerprint()
+    # Query ids are the queries' positions in the workload.
+    query_words = _fnv_words([v for i, (q, t) in enumerate(zip(qs, tiles)) for v in (i, q.source, q.target, len(t), *t)])
+    fp = shootout.Fingerprint(tuple(int(w) for w in graph_words), query_words, len(qs))
+    workload = shootout.Workload(shootout.WORKLOAD_SYNTHETIC, 1 + sum(cfg["tiles"]) * 1000 + cfg["queries"],
+                                 args.workload)
+    configs = [shootout.Tuning(i, engine, shootout.PER_ROUND, 1, 256, maximum_rounds=1 << 20)
+               for i, engine in enumerate((bfn.JACOBI_PULL, bfn.DENSE_CHAOTIC_PUSH, bfn.FRONTIER_PUSH))]
+    features = []
+    for i, (q, t) in enumerate(zip(qs, tiles)):
+        sel = np.zeros(arrays["tile_count"], bool)
+        sel[np.asarray(t, np.int64)] = True
+        one = bfn.Batch([[q.source]], [[q.target]], [t])
+        ws = bfn.Workspace(dev)
+        ws.submit(dg, one, bfn.run_options(engine=bfn.JACOBI_PULL, control_mode=bfn.PER_ROUND,
+                                           rounds_per_chunk=1), bfn.tuning(tile_sweeps=1))
+        _, per, _ = ws.status()
+        ws.close()
+        features.append(shootout.QueryFeatures(
+            i, int(tile_vertices[sel].sum()), int(np.count_nonzero(sel[src_tile] & sel[dst_tile])),
+            1, 1, max(1, int(per[0].rounds_completed))))
+    bins = shootout.quartile_bins(features)
+    manifest = shootout.Manifest(fp, workload, cfg["query_seed"], 0, 1, args.reps, len(qs),
+                                 list(zip(features, bins)))
+    catalog = shootout.Catalog(fp, workload, configs)
+    samples, ordinal = [], 0
+    for c in configs:
+        for i, (q, t) in enumerate(zip(qs, tiles)):
+            one = bfn.Batch([[q.source]], [[q.target]], [t])
+            ws = bfn.Workspace(dev)
+            ws.submit(dg, one, bfn.run_options(engine=c.engine, control_mode=c.control_mode,
+                                               rounds_per_chunk=c.rounds_per_chunk, block_size=c.block_size,
+                                               maximum_rounds=c.maximum_rounds),
+                      bfn.tuning(tile_sweeps=1))
+            for rep in range(args.reps):
+                t0 = time.perf_counter()
+                ws.rerun()
+                wall = (time.perf_counter() - t0) * 1e3
+                st, per, work = ws.status()
+                timeline = ws.timing()[1]
+                s = shootout.Sample(fp, workload, shootout.RUN_TIMING, ordinal, rep, i,
+                                    c.configuration_id, c.engine, c.control_mode, correctness_passed=True)
+                s.status.update(final_distance_slot=st.final_distance_slot, converged=st.converged,
+                                rounds_completed=st.rounds_completed, reached_target_mask=st.reached_target_mask,
+                                bounding_box_miss_mask=st.bounding_box_miss_mask,
+                                valid_lane_mask=st.valid_lane_mask, active_lane_mask=st.active_lane_mask,
+                                converged_lane_mask=st.converged_lane_mask, stop_reason=st.stop_reason,
+                                error_bits=st.error_bits)
+                for k in shootout.WORK_FIELDS:
+                    s.work[k] = int(getattr(work, k))
+                s.timing["timeline"] = shootout.evidence(timeline)
+                s.timing["wall"] = shootout.evidence(wall)
+                samples.append(s)
+                ordinal += 1
+            ws.close()
+    return manifest, catalog, samples
+
+
+def write_shootout(args, cfg, dev, hg, dg, qs, tiles):
+    manifest, catalog, samples = shootout_campaign(args, cfg, dev, hg, dg, qs, tiles)
+    out = os.path.join(args.out, f"shootout_{args.workload}")
+    os.makedirs(out, exist_ok=True)
+    texts = {"manifest.v2.tsv": shootout.serialize_manifest(manifest),
+             "catalog.v2.tsv": shootout.serialize_catalog(catalog),
+             "samples.v2.tsv": shootout.serialize_samples(manifest, catalog.tunings, samples,
+                                                          shootout.RUN_TIMING)}
+    for name, text in texts.items():
+        with open(os.path.join(out, name), "w", newline="\n") as f:
+            f.write(text)
+    # Replay check: the files read back to the same records.
+    assert shootout.deserialize_manifest(texts["manifest.v2.tsv"]) == manifest
+    assert shootout.samples_equal(shootout.deserialize_samples(manifest, catalog.tunings, texts["samples.v2.tsv"],
+                                                               shootout.RUN_TIMING), samples)
+    medians = {}
+    for c in catalog.tunings:
+        walls = [s.timing["wall"][1] for s in samples if s.configuration_id == c.configuration_id]
+        medians[c.configuration_id] = statistics.median(walls)
+    best = min(medians, key=medians.get)
+    decision = {"workload": args.workload, "selected_configuration": best,
+                "engine": bfn.ENGINE_NAMES[catalog.tunings[best].engine],
+                "median_wall_ms": {bfn.ENGINE_NAMES[c.engine]: round(medians[c.configuration_id], 4)
+                                   for c in catalog.tunings},
+                "samples": len(samples), "directory": os.path.relpath(out, HERE)}
+    log("shootout:", json.dumps(decision))
+    print(json.dumps(decision))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="c3_batched_128x128x512_q256", choices=sorted(WORKLOADS))
@@ -109,6 +225,8 @@ def main():
     ap.add_argument("--blocks", default="128,256,512")
     ap.add_argument("--quick", action="store_true")
     ap.add_argument("--out", default=os.path.join(HERE, "tuning"))
+    ap.add_argument("--shootout", action="store_true",
+                    help="run the per-query shootout campaign and write its TSV records")
     args = ap.parse_args()
     cfg = WORKLOADS[args.workload]
 
@@ -119,6 +237,9 @@ def main():
                               side_max=cfg["side"][1], unbounded_fraction=cfg["unbounded"])
     tiles = [hg.query_tiles(q) for q in qs]
     batch = bfn.Batch([[q.source] for q in qs], [[q.target] for q in qs], tiles)
+    if args.shootout:
+        write_shootout(args, cfg, dev, hg, dg, qs, tiles)
+        return
 
     blocks = [int(b) for b in args.blocks.split(",")]
     engines = (bfn.JACOBI_PULL, bfn.DENSE_CHAOTIC_PUSH, bfn.FRONTIER_PUSH)
