@@ -20,6 +20,7 @@
 #include "bfnew/hip/batched_expansion.hpp"
 #include "bfnew/hip/engines.hpp"
 #include "bfnew/hip/runtime.hpp"
+#include "bfnew/no_congestion_pipeline.hpp"
 #include "bfnew/query.hpp"
 
 #include <algorithm>
@@ -871,6 +872,144 @@ void host_mode() {
           a.edge_id_bytes == 12U && a.total_bytes == 72U);
   }
 
+  // The no-congestion pipeline over the host runner: results, accounting,
+  // ledger and quality sample.
+  {
+    const auto qs = f.subset({kImmediate, kTwoRing, kLongEdge, kSpill, kUnreachable, kSecondMiss, kMultiSource});
+    BatchedExpansionOptions o = options_of(EngineKind::frontier_push, one_ring_expansion());
+    o.enable_compact_paths = 1U;
+    HostRunner runner{g, o.run_options, true};
+    NoCongestionPipelineOptions po;
+    po.quality_sample_query_count = 5U;
+    po.quality_selection_seed = 42U;
+    po.cold_artifact_load = measured_host_stage(10U);
+    po.graph_upload = measured_host_stage(20U);
+    po.cold_execution = measured_host_stage(30U);
+    const auto r = run_no_congestion_pipeline(g, f.directory, f.tile_runs, qs, o, std::ref(runner), po);
+    const NoCongestionResultAccounting& a = r.result_accounting;
+    CHECK(r.compact_results.size() == 7U && a.query_count == 7U && a.target_summary_count == 7U);
+    CHECK(a.complete_path_count == 6U && a.unreachable_target_count == 1U && a.terminal_failure_target_count == 0U);
+    CHECK(a.modeled_batch_status_count == r.expansion.metrics.batches_executed);
+    CHECK(a.device_transfer_evidence == CompactStageTimingEvidence::measured &&
+          a.actual_status_bytes == a.modeled_batch_status_bytes);
+    std::uint64_t payload = 0;
+    for (const CompactQueryResult& cr : r.compact_results) {
+      CHECK(validate_compact_query_result(g, outcome_of(r.expansion, cr.query_id.value()).final_query, cr).ok());
+      for (const CompactTargetPath& t : cr.targets) payload += 28U + 8U * t.vertices.size() + 4U * t.edge_ids.size();
+    }
+    CHECK(a.final_result_serialization.total_bytes == payload);
+    CHECK(validate_no_congestion_stage_ledger(r.timing) == NoCongestionStageLedgerError::none);
+    CHECK(r.timing.cold_pipeline.host_evidence == PipelineTimingEvidence::measured &&
+          r.timing.cold_pipeline.host_nanoseconds == 60U);
+    CHECK(r.timing.warm_all_query.host_nanoseconds == r.expansion.metrics.total_nanoseconds);
+    // Quality: every sampled complete path is the unbounded shortest path here.
+    CHECK(r.quality.sampled_query_count == 5U && r.quality.sampled_query_ids.size() == 5U);
+    CHECK(r.quality.finite_target_pairs == r.quality.observations.size() && r.quality.finite_target_pairs >= 4U);
+    for (const CompactPathQualityObservation& ob : r.quality.observations)
+      CHECK(ob.cost_inflation_ratio == 1.0 && ob.path_length_inflation_ratio == 1.0 &&
+            ob.absolute_cost_inflation == 0.0 && ob.absolute_path_length_inflation == 0);
+    CHECK(r.quality.cost_ratio_p50 == 1.0 && r.quality.cost_ratio_max == 1.0);
+    // The sample does not depend on input order.
+    std::vector<RouteQuery> reversed(qs.rbegin(), qs.rend());
+    const auto r2 = run_no_congestion_pipeline(g, f.directory, f.tile_runs, reversed, o, std::ref(runner), po);
+    CHECK(r2.quality.sampled_query_ids == r.quality.sampled_query_ids &&
+          r2.quality.observations == r.quality.observations);
+    // A bounded detour shows as inflation: the two-ring route forced through
+    // a longer region-limited path is compared with its unbounded one.
+    std::vector<ExpansionQueryOutcome> outs{outcome_of(r.expansion, kTwoRing)};
+    std::vector<CompactQueryResult> res{r.compact_results[1]};
+    CHECK(res[0].query_id.value() == kTwoRing);
+    res[0].targets[0].summary.distance = 3.0F;  // unchanged: ratio 1
+    CHECK(sample_compact_path_quality(g, outs, res, 1U, 7U).cost_ratio_max == 1.0);
+
+    // Rejections.
+    CHECK(throws<std::invalid_argument>([&] {
+      BatchedExpansionOptions no_paths = o;
+      no_paths.enable_compact_paths = 0U;
+      HostRunner plain{g, o.run_options};
+      (void)run_no_congestion_pipeline(g, f.directory, f.tile_runs, qs, no_paths, std::ref(plain), po);
+    }));
+    std::vector<CompactQueryResult> swapped = r.compact_results;
+    std::swap(swapped[0], swapped[1]);
+    CHECK(throws<std::invalid_argument>([&] { (void)measure_no_congestion_result_transfer(swapped, 1U); }));
+    std::vector<CompactQueryResult> broken = r.compact_results;
+    broken[0].targets[0].edge_ids.push_back(EdgeId{0U});
+    CHECK(throws<std::invalid_argument>([&] { (void)measure_no_congestion_result_transfer(broken, 1U); }));
+    BatchedExpansionMetrics lying = r.expansion.metrics;
+    lying.compact_status_bytes += 48U;
+    CHECK(throws<std::invalid_argument>([&] {
+      (void)measure_no_congestion_result_transfer(r.compact_results, r.expansion.metrics.batches_executed, &lying);
+    }));
+    CHECK(throws<std::invalid_argument>(
+        [&] { (void)make_failed_compact_query_result(qs[0], ExpansionQueryDisposition::reached); }));
+    const CompactQueryResult failed = make_failed_compact_query_result(qs[1], ExpansionQueryDisposition::expansion_limit);
+    CHECK(validate_compact_query_result(g, qs[1], failed).ok());
+
+    // Ledger rules, one per error code.
+    using E = NoCongestionStageLedgerError;
+    const auto ledger = [&](auto&& edit) {
+      NoCongestionStageLedger l = r.timing;
+      edit(l);
+      return validate_no_congestion_stage_ledger(l);
+    };
+    CHECK(ledger([](auto& l) { l.sssp.host_evidence = static_cast<PipelineTimingEvidence>(9); }) == E::invalid_evidence);
+    CHECK(ledger([](auto& l) { l.graph_upload = PipelineStageTiming{PipelineTimingEvidence::unavailable, 5U}; }) ==
+          E::unavailable_host_has_value);
+    CHECK(ledger([](auto& l) { l.sssp.device_milliseconds = 1.0; }) == E::unavailable_device_has_value);
+    CHECK(ledger([](auto& l) {
+            l.sssp.device_evidence = PipelineTimingEvidence::measured;
+            l.sssp.device_milliseconds = -1.0;
+          }) == E::invalid_device_duration);
+    CHECK(ledger([](auto& l) {
+            l.batch_planning.device_evidence = PipelineTimingEvidence::measured;
+            l.batch_planning.device_milliseconds = 1.0;
+          }) == E::cpu_only_device_evidence);
+    CHECK(ledger([](auto& l) { l.warm_all_query = PipelineStageTiming{}; }) == E::warm_evidence_mismatch);
+    CHECK(ledger([](auto& l) { l.warm_all_query.host_nanoseconds += 1U; }) == E::warm_sum_mismatch);
+    CHECK(ledger([](auto& l) { l.cold_pipeline = PipelineStageTiming{}; }) == E::cold_evidence_mismatch);
+    CHECK(ledger([](auto& l) { l.cold_pipeline.host_nanoseconds = 61U; }) == E::cold_sum_mismatch);
+    CHECK(ledger([](auto& l) {
+            l.cold_artifact_load.host_nanoseconds = ~std::uint64_t{0};
+            l.cold_pipeline.host_nanoseconds = 0U;
+          }) == E::cold_sum_overflow);
+    CHECK(throws<std::invalid_argument>([&] {
+      (void)make_no_congestion_stage_ledger({}, {}, measured_host_stage(5U), {}, {}, {}, {}, {},
+                                            measured_host_stage(4U), {});
+    }));
+  }
+
+  // Path quality sees region-bound detours: s -> b -> t (cost 10, b inside
+  // the terminal rectangle) versus s -> a -> t (cost 2, a three tiles away).
+  {
+    const ResourceClassId rc{1U};
+    std::vector<VertexMetadata> v{VertexMetadata::located(0, 0, rc), VertexMetadata::located(20, 0, rc),
+                                  VertexMetadata::located(10, 30, rc), VertexMetadata::located(10, 0, rc)};
+    std::vector<EdgeInputRecord> e{edge(0, 2, 1.0F, 1), edge(2, 1, 1.0F, 2), edge(0, 3, 5.0F, 3),
+                                   edge(3, 1, 5.0F, 4)};
+    const UniformGridPartitioner grid{SpatialOrderConfig{0, 0, 10U, 10U}};
+    const PartitionedGraph pg = grid.partition(InputGraph{std::move(v), std::move(e)});
+    const WeightedGraph& dg = pg.graph;
+    const auto to_new = dg.old_to_new();
+    const std::vector<VertexId> s1{to_new[0]}, t1{to_new[1]};
+    const std::vector<RouteQuery> dq{make_route_query(QueryId{5U}, dg, s1, t1)};
+    BatchedExpansionOptions o = options_of(EngineKind::jacobi_pull, one_ring_expansion());
+    o.enable_compact_paths = 1U;
+    HostRunner runner{dg, o.run_options, true};
+    NoCongestionPipelineOptions po;
+    po.quality_sample_query_count = 1U;
+    const auto r = run_no_congestion_pipeline(dg, build_tile_directory(dg), build_tile_run_layout(dg), dq, o,
+                                              std::ref(runner), po);
+    CHECK(r.expansion.queries.front().reached() && r.expansion.queries.front().total_expansions == 0U);
+    CHECK(r.quality.observations.size() == 1U);
+    if (r.quality.observations.size() == 1U) {
+      const CompactPathQualityObservation& ob = r.quality.observations.front();
+      CHECK(ob.bounded_distance == 10.0F && ob.unbounded_distance == 2.0F && ob.cost_inflation_ratio == 5.0 &&
+            ob.absolute_cost_inflation == 8.0 && ob.bounded_path_length == 2U && ob.unbounded_path_length == 2U &&
+            ob.path_length_inflation_ratio == 1.0);
+      CHECK(r.quality.cost_ratio_p50 == 5.0 && r.quality.absolute_cost_inflation_max == 8.0);
+    }
+  }
+
   // The spill tile joins a region only through a real directory adjacency.
   {
     const ResourceClassId rc{1U};
@@ -991,6 +1130,26 @@ int gpu_mode() {
         CHECK_MSG(x.final_query == y.final_query && x.disposition == y.disposition && x.attempts == y.attempts &&
                       x.compact_paths == y.compact_paths,
                   tag + " pass runner paths of query " + std::to_string(y.final_query.query_id.value()));
+      }
+      // The no-congestion pipeline on the device runner: the host runner's
+      // results and quality, measured device bytes.
+      {
+        NoCongestionPipelineOptions po;
+        po.quality_sample_query_count = 4U;
+        po.quality_selection_seed = 9U;
+        hip::PassBatchedExpansionExecutor pipe_exec(g, f.tile_runs, resident, ws, stream, run,
+                                                    hip::BatchedExpansionTransferMode::compact_paths);
+        HostRunner host_pipe{g, run, true};
+        const auto dev = run_no_congestion_pipeline(g, f.directory, f.tile_runs, queries, o, std::ref(pipe_exec), po);
+        const auto cpu = run_no_congestion_pipeline(g, f.directory, f.tile_runs, queries, o, std::ref(host_pipe), po);
+        CHECK_MSG(dev.compact_results == cpu.compact_results, tag + " pipeline");
+        CHECK_MSG(dev.quality.observations == cpu.quality.observations, tag + " pipeline");
+        CHECK_MSG(dev.result_accounting.device_transfer_evidence == CompactStageTimingEvidence::measured &&
+                      dev.result_accounting.final_result_serialization ==
+                          cpu.result_accounting.final_result_serialization,
+                  tag + " pipeline");
+        CHECK_MSG(validate_no_congestion_stage_ledger(dev.timing) == NoCongestionStageLedgerError::none,
+                  tag + " pipeline");
       }
       for (std::size_t i = 0; i < want_p.queries.size(); ++i) {
         const ExpansionQueryOutcome &x = got_p.queries[i], &y = want_p.queries[i];
