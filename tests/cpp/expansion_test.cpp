@@ -21,6 +21,7 @@
 #include "bfnew/hip/engines.hpp"
 #include "bfnew/hip/runtime.hpp"
 #include "bfnew/no_congestion_pipeline.hpp"
+#include "bfnew/workload_artifact.hpp"
 #include "bfnew/query.hpp"
 
 #include <algorithm>
@@ -28,6 +29,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <filesystem>
 #include <functional>
 #include <limits>
 #include <map>
@@ -35,6 +37,8 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+
+#include <unistd.h>
 
 using namespace bfnew;
 
@@ -97,26 +101,11 @@ EdgeInputRecord edge(std::uint32_t s, std::uint32_t d, float w, std::uint32_t re
   return e;
 }
 
+InputGraph fixture_input();
+
 Fixture make_fixture() {
-  const ResourceClassId rc{1U};
-  const auto at = [rc](std::int32_t x, std::int32_t y) { return VertexMetadata::located(x, y, rc); };
-  std::vector<VertexMetadata> v{
-      at(40, 0),   at(50, 0),   at(60, 0),                 // 0-2 immediate chain
-      at(0, 0),    at(20, 0),   at(0, 10),   at(10, 20),   // 3-6 two-ring northern detour
-      at(80, 0),   at(100, 0),  at(80, 40),                // 7-9 jump to a fourth-ring vertex
-      at(120, 0),  at(120, 20), VertexMetadata::unlocated(rc),  // 10-12 spill target
-      at(160, 0),  at(180, 0),                             // 13-14 unreachable
-      at(220, 0),  at(240, 0),  at(220, -10), at(230, -30),  // 15-18 southern detour
-      at(80, 10),  at(80, 20),  at(80, 30),  at(120, 10), at(220, -20),  // 19-23 ring fillers
-      at(1, 0)};                                           // 24 second northern source
-  std::vector<EdgeInputRecord> e{
-      edge(0, 1, 1.0F, 17000),  edge(1, 2, 1.0F, 17001),  edge(3, 5, 1.0F, 17002),
-      edge(5, 6, 1.0F, 17003),  edge(6, 4, 1.0F, 17004),  edge(7, 9, 2.0F, 17005),
-      edge(9, 8, 3.0F, 17006),  edge(10, 11, 4.0F, 17007), edge(11, 12, 5.0F, 17008),
-      edge(15, 17, 1.0F, 17009), edge(17, 18, 1.0F, 17010), edge(18, 16, 1.0F, 17011),
-      edge(24, 5, 0.5F, 17012)};
   const UniformGridPartitioner grid{SpatialOrderConfig{0, 0, 10U, 10U}};
-  Fixture f{grid.partition(InputGraph{std::move(v), std::move(e)}), {}, {}, {}};
+  Fixture f{grid.partition(fixture_input()), {}, {}, {}};
   const WeightedGraph& g = f.graph();
   const auto to_new = g.old_to_new();
   const auto route = [&](std::uint32_t id, std::vector<std::uint32_t> src, std::uint32_t tgt) {
@@ -135,6 +124,27 @@ Fixture make_fixture() {
   f.directory = build_tile_directory(g);
   f.tile_runs = build_tile_run_layout(g);
   return f;
+}
+
+InputGraph fixture_input() {
+  const ResourceClassId rc{1U};
+  const auto at = [rc](std::int32_t x, std::int32_t y) { return VertexMetadata::located(x, y, rc); };
+  std::vector<VertexMetadata> v{
+      at(40, 0),   at(50, 0),   at(60, 0),                 // 0-2 immediate chain
+      at(0, 0),    at(20, 0),   at(0, 10),   at(10, 20),   // 3-6 two-ring northern detour
+      at(80, 0),   at(100, 0),  at(80, 40),                // 7-9 jump to a fourth-ring vertex
+      at(120, 0),  at(120, 20), VertexMetadata::unlocated(rc),  // 10-12 spill target
+      at(160, 0),  at(180, 0),                             // 13-14 unreachable
+      at(220, 0),  at(240, 0),  at(220, -10), at(230, -30),  // 15-18 southern detour
+      at(80, 10),  at(80, 20),  at(80, 30),  at(120, 10), at(220, -20),  // 19-23 ring fillers
+      at(1, 0)};                                           // 24 second northern source
+  std::vector<EdgeInputRecord> e{
+      edge(0, 1, 1.0F, 17000),  edge(1, 2, 1.0F, 17001),  edge(3, 5, 1.0F, 17002),
+      edge(5, 6, 1.0F, 17003),  edge(6, 4, 1.0F, 17004),  edge(7, 9, 2.0F, 17005),
+      edge(9, 8, 3.0F, 17006),  edge(10, 11, 4.0F, 17007), edge(11, 12, 5.0F, 17008),
+      edge(15, 17, 1.0F, 17009), edge(17, 18, 1.0F, 17010), edge(18, 16, 1.0F, 17011),
+      edge(24, 5, 0.5F, 17012)};
+  return InputGraph{std::move(v), std::move(e)};
 }
 
 BatchedExpansionOptions options_of(EngineKind engine, ExpansionSchedulePolicy schedule,
@@ -1161,6 +1171,51 @@ int gpu_mode() {
                     tag + " paths of query " + std::to_string(y.final_query.query_id.value()));
       }
     }
+  }
+
+  // From files: the fixture written as BFGRPH07 / BFQRY007, read back,
+  // rebuilt, uploaded and run gives the in-memory run's outcomes.
+  {
+    const auto dir = std::filesystem::temp_directory_path() / ("bfnew_expansion_" + std::to_string(::getpid()));
+    std::vector<MappedRouteQuery> mapped;
+    for (const RouteQuery& q : queries) mapped.push_back(MappedRouteQuery{"net", q, {}, {}});
+    for (MappedRouteQuery& m : mapped) {
+      for (VertexId v : m.query.source_terminals) {
+        MappedTerminal t{"site", "pin", g.new_to_old()[v.value()], v, g.vertices()[v.value()].x,
+                         g.vertices()[v.value()].y, g.vertices()[v.value()].has_location, g.owner_tiles()[v.value()]};
+        m.source_terminals.push_back(t);
+      }
+      for (VertexId v : m.query.target_terminals) {
+        MappedTerminal t{"site", "pin", g.new_to_old()[v.value()], v, g.vertices()[v.value()].x,
+                         g.vertices()[v.value()].y, g.vertices()[v.value()].has_location, g.owner_tiles()[v.value()]};
+        m.target_terminals.push_back(t);
+      }
+    }
+    write_graph_artifact(dir / "graph.bfgrph", fixture_input());
+    write_query_artifact(dir / "queries.bfqry", mapped);
+    const UniformGridPartitioner grid{SpatialOrderConfig{0, 0, 10U, 10U}};
+    const PartitionedGraph loaded = grid.partition(read_graph_artifact(dir / "graph.bfgrph"));
+    const WeightedGraph& lg = loaded.graph;
+    std::vector<RouteQuery> lq;
+    for (const MappedRouteQuery& m : read_query_artifact(dir / "queries.bfqry", lg)) lq.push_back(m.query);
+    std::filesystem::remove_all(dir);
+    const TileRunLayout64 lruns = build_tile_run_layout(lg);
+    hip::ResidentDeviceGraph lresident;
+    lresident.upload_once_async(hip::make_resident_graph_plan(build_device_graph_layout32(lg, lruns)), stream);
+    lresident.wait_until_ready(stream);
+    BatchedExpansionOptions o = options_of(EngineKind::frontier_push, hybrid_margin_expansion(2U));
+    o.enable_compact_paths = 1U;
+    hip::PassBatchedExpansionExecutor from_files(lg, lruns, lresident, ws, stream, o.run_options,
+                                                 hip::BatchedExpansionTransferMode::compact_paths);
+    hip::PassBatchedExpansionExecutor in_memory(g, f.tile_runs, resident, ws, stream, o.run_options,
+                                                hip::BatchedExpansionTransferMode::compact_paths);
+    const auto a = run_batched_expansion(lg, build_tile_directory(lg), lruns, lq, o, std::ref(from_files));
+    const auto b = run_batched_expansion(g, f.directory, f.tile_runs, queries, o, std::ref(in_memory));
+    CHECK(lq == queries && a.queries.size() == b.queries.size());
+    for (std::size_t i = 0; i < std::min(a.queries.size(), b.queries.size()); ++i)
+      CHECK(a.queries[i].final_query == b.queries[i].final_query &&
+            a.queries[i].disposition == b.queries[i].disposition &&
+            a.queries[i].compact_paths == b.queries[i].compact_paths);
   }
   return 0;
 }
