@@ -2,6 +2,8 @@
 // (reference rule: src/batch_plan.cpp:271-543; TSV image :870-929).
 #include "bfnew/batch_plan.hpp"
 
+#include "bfnew/device_layout.hpp"
+
 #include <algorithm>
 #include <bit>
 #include <iterator>
@@ -416,6 +418,14 @@ SelectedRegionIndex::SelectedRegionIndex(const WeightedGraph& graph,
       in_pair_tiles_[at] = static_cast<std::uint32_t>(from);
       in_pair_edges_[at] = pair_edges_[i];
     }
+}
+
+const SelectedRegionIndex& graph_region_index(const WeightedGraph& graph) {
+  WeightedGraphMemo& memo = graph.memo();
+  std::call_once(memo.region_index_once, [&] {
+    memo.region_index = std::make_shared<const SelectedRegionIndex>(graph, graph_tile_runs(graph));
+  });
+  return *static_cast<const SelectedRegionIndex*>(memo.region_index.get());
 }
 
 SelectedRegionIndex::TilePairs SelectedRegionIndex::pairs_from(std::uint32_t tile) const noexcept {

@@ -206,11 +206,18 @@ DeviceGraphFingerprint fingerprint_device_graph_layout32(const DeviceGraphLayout
   return f.finish();
 }
 
+const TileRunLayout64& graph_tile_runs(const WeightedGraph& graph) {
+  WeightedGraphMemo& memo = graph.memo();
+  std::call_once(memo.tile_runs_once,
+                 [&] { memo.tile_runs = std::make_shared<const TileRunLayout64>(build_tile_run_layout(graph)); });
+  return *static_cast<const TileRunLayout64*>(memo.tile_runs.get());
+}
+
 DeviceGraphFingerprint fingerprint_device_graph_source32(const WeightedGraph& graph,
                                                          const TileRunLayout64& tile_runs) {
   // The tile runs are a function of the graph, so the graph's memo holds the
   // value once runs equal to the graph's own have been seen.
-  if (tile_runs != build_tile_run_layout(graph))
+  if (tile_runs != graph_tile_runs(graph))
     throw std::invalid_argument{"device graph fingerprint: tile runs are not the graph's"};
   WeightedGraphMemo& memo = graph.memo();
   std::call_once(memo.device_fingerprint_once, [&] {

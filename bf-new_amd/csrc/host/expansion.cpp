@@ -570,8 +570,8 @@ BatchedExpansionRunResult run_batched_expansion(const WeightedGraph& graph, cons
     throw std::invalid_argument{"batched expansion: invalid options"};
   if (!runner) throw std::invalid_argument{"batched expansion: no batch runner"};
   if (!graph.has_spatial_ordering()) throw std::invalid_argument{"batched expansion: graph is not spatially ordered"};
-  if (const char* why = validate_weighted_graph_message(graph); why && !std::string_view{why}.empty())
-    throw std::invalid_argument{std::string{"batched expansion: invalid graph: "} + why};
+  if (const std::string& why = weighted_graph_validation(graph); !why.empty())
+    throw std::invalid_argument{"batched expansion: invalid graph: " + why};
   if (!directory.built_from(graph) && !directory_matches(directory, build_tile_directory(graph)))
     throw std::invalid_argument{"batched expansion: the tile directory is not the graph's"};
   // Also checks that the tile runs are the graph's (std::invalid_argument
@@ -590,7 +590,9 @@ BatchedExpansionRunResult run_batched_expansion(const WeightedGraph& graph, cons
     return out;
   }
 
-  const SelectedRegionIndex regions{graph, tile_runs};
+  // tile_runs equal the graph's own (checked above), so the graph's memoised
+  // index serves.
+  const SelectedRegionIndex& regions = graph_region_index(graph);
   std::vector<TileId> every_tile(graph.tile_coordinates().size());
   for (std::size_t t = 0; t < every_tile.size(); ++t) every_tile[t] = TileId{static_cast<std::uint32_t>(t)};
 

@@ -366,4 +366,13 @@ bool TileDirectory::built_from(const WeightedGraph& graph) const noexcept {
   return memo && memo.get() == &graph.memo();
 }
 
+const std::string& weighted_graph_validation(const WeightedGraph& graph) {
+  WeightedGraphMemo& memo = graph.memo();
+  std::call_once(memo.validation_once, [&] {
+    const char* why = validate_weighted_graph_message(graph);
+    memo.validation_message = why ? why : "";
+  });
+  return memo.validation_message;
+}
+
 }  // namespace bfnew

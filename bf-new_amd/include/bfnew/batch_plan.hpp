@@ -73,6 +73,10 @@ class SelectedRegionIndex {
   std::vector<std::uint64_t> in_pair_edges_;
 };
 
+// The index of the graph and its own tile runs, built once per graph and
+// kept in its memo.
+[[nodiscard]] const SelectedRegionIndex& graph_region_index(const WeightedGraph& graph);
+
 inline constexpr std::uint32_t invalid_batch_query_index = 0xFFFFFFFFu;
 inline constexpr QueryId invalid_batch_query_id{0xFFFFFFFFu};
 

@@ -24,6 +24,9 @@ struct TileRunLayout64 {
 
 [[nodiscard]] TileRunLayout64 build_tile_run_layout(const WeightedGraph& graph);
 
+// build_tile_run_layout once per graph, kept in its memo.
+[[nodiscard]] const TileRunLayout64& graph_tile_runs(const WeightedGraph& graph);
+
 // Host staging image of the immutable device graph.
 struct DeviceGraphLayout32 {
   std::uint32_t vertex_count{};

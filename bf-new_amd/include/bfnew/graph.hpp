@@ -10,6 +10,7 @@
 #include <memory>
 #include <mutex>
 #include <span>
+#include <string>
 #include <vector>
 
 namespace bfnew {
@@ -81,6 +82,14 @@ class SyntheticTileGraph;
 struct WeightedGraphMemo {
   std::once_flag device_fingerprint_once;
   std::uint64_t device_fingerprint[2]{};
+  std::once_flag validation_once;
+  std::string validation_message;  // empty when the graph is valid
+  // Derived layouts built once per graph by the libraries that own their
+  // types (tile runs, selected-region index); typed where they are used.
+  std::once_flag tile_runs_once;
+  std::shared_ptr<const void> tile_runs;
+  std::once_flag region_index_once;
+  std::shared_ptr<const void> region_index;
 };
 
 class WeightedGraph {
@@ -152,5 +161,9 @@ struct PartitionedGraph {
 // transpose agreement, permutation round trip and tile spans.  Returns an
 // empty string when valid, else a description of the first violation.
 [[nodiscard]] const char* validate_weighted_graph_message(const WeightedGraph& graph);
+
+// validate_weighted_graph_message, checked once per graph (graphs are
+// immutable) and kept in its memo.
+[[nodiscard]] const std::string& weighted_graph_validation(const WeightedGraph& graph);
 
 }  // namespace bfnew
