@@ -367,7 +367,7 @@ enum Column : int {
 // ---------------------------------------------------------------------------
 SelectedRegionIndex::SelectedRegionIndex(const WeightedGraph& graph,
                                          const TileRunLayout64& tile_runs)
-    : graph_{&graph} {
+    : graph_memo_{graph.memo_handle()} {
   const auto offsets = graph.tile_vertex_offsets();
   const auto owners = graph.owner_tiles();
   const std::size_t V = graph.vertex_count();

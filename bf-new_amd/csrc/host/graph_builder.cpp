@@ -362,8 +362,8 @@ PartitionedGraph UniformGridPartitioner::partition(const InputGraph& input) cons
 }
 
 bool TileDirectory::built_from(const WeightedGraph& graph) const noexcept {
-  const auto memo = built_from_.lock();
-  return memo && memo.get() == &graph.memo();
+  const auto other = graph.memo_handle();
+  return !built_from_.expired() && !built_from_.owner_before(other) && !other.owner_before(built_from_);
 }
 
 const std::string& weighted_graph_validation(const WeightedGraph& graph) {

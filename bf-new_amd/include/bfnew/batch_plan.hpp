@@ -28,6 +28,7 @@
 #include <array>
 #include <cstddef>
 #include <cstdint>
+#include <memory>
 #include <span>
 #include <string>
 #include <vector>
@@ -43,8 +44,11 @@ class SelectedRegionIndex {
   [[nodiscard]] std::uint32_t tile_count() const noexcept {
     return static_cast<std::uint32_t>(vertex_counts_.size());
   }
+  // True for the graph the index was built from and its copies (which share
+  // its memo and hold the same data).
   [[nodiscard]] bool matches_graph(const WeightedGraph& graph) const noexcept {
-    return graph_ == &graph;
+    const auto other = graph.memo_handle();
+    return !graph_memo_.owner_before(other) && !other.owner_before(graph_memo_);
   }
   // Throws std::out_of_range for a tile outside the graph; tiles must be
   // sorted and unique.
@@ -63,7 +67,7 @@ class SelectedRegionIndex {
   [[nodiscard]] TilePairs pairs_into(std::uint32_t tile) const noexcept;
 
  private:
-  const WeightedGraph* graph_;
+  std::weak_ptr<const WeightedGraphMemo> graph_memo_;
   std::vector<std::uint64_t> vertex_counts_;
   std::vector<std::uint64_t> pair_offsets_;   // [tile_count + 1]
   std::vector<std::uint32_t> pair_tiles_;     // destination tile, ascending per source tile

@@ -10,6 +10,7 @@
 #include <memory>
 #include <mutex>
 #include <span>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -114,7 +115,11 @@ class WeightedGraph {
   [[nodiscard]] std::span<const TileCoordinate> tile_coordinates() const noexcept { return tile_coordinates_; }
   [[nodiscard]] std::span<const EdgeOffset> tile_vertex_offsets() const noexcept { return tile_vertex_offsets_; }
   [[nodiscard]] const SpatialOrderConfig& spatial_order_config() const noexcept { return config_; }
-  [[nodiscard]] WeightedGraphMemo& memo() const noexcept { return *memo_; }
+  // Throws std::logic_error on a moved-from graph.
+  [[nodiscard]] WeightedGraphMemo& memo() const {
+    if (!memo_) throw std::logic_error{"WeightedGraph: moved-from graph"};
+    return *memo_;
+  }
   [[nodiscard]] std::weak_ptr<const WeightedGraphMemo> memo_handle() const noexcept { return memo_; }
 
  private:

@@ -1047,6 +1047,23 @@ void host_mode() {
     CHECK(oc.final_distances.size() == sg.vertex_count() && same_bits(oc.final_distances[t[0].value()], 3.0F));
     // A directory built from another graph is rejected.
     CHECK(dir.built_from(sg) && !dir.built_from(g));
+    // A copy holds the same data: the directory and the memoised region
+    // index serve it, and the run on the copy equals the run on the
+    // original.  A moved graph keeps them; the moved-from graph refuses.
+    {
+      const WeightedGraph copy = sg;
+      CHECK(dir.built_from(copy) && graph_region_index(copy).matches_graph(copy) &&
+            graph_region_index(copy).matches_graph(sg) && !graph_region_index(sg).matches_graph(g));
+      HostRunner copy_runner{copy, o.run_options};
+      const auto rc2 = run_batched_expansion(copy, dir, runs, std::vector<RouteQuery>{q}, o, std::ref(copy_runner));
+      CHECK(rc2.trace == r.trace && rc2.queries.front().final_query == oc.final_query &&
+            rc2.queries.front().scheduled_expansions == oc.scheduled_expansions);
+      WeightedGraph moved_from = sg;
+      const WeightedGraph moved = std::move(moved_from);
+      CHECK(dir.built_from(moved) && graph_region_index(moved).matches_graph(moved));
+      CHECK(!dir.built_from(moved_from));  // NOLINT(bugprone-use-after-move): the refusal is the check
+      CHECK(throws<std::logic_error>([&] { (void)moved_from.memo(); }));  // NOLINT(bugprone-use-after-move)
+    }
     const BatchedExpansionOptions fo = options_of(EngineKind::jacobi_pull, one_ring_expansion());
     HostRunner fixture_runner{g, fo.run_options};
     CHECK(throws<std::invalid_argument>([&] {
