@@ -118,7 +118,9 @@ int main(int argc, char** argv) {
     cfg.seed = a.seed;
     const SyntheticTileGraph gen = SyntheticTileGraph::generate(cfg);
     const WeightedGraph g = gen.weighted_graph();
-    const TileRunLayout64 runs = build_tile_run_layout(g);
+    // The graph's memoised runs: the orchestrator's per-call check that the
+    // runs are the graph's is then an identity test.
+    const TileRunLayout64& runs = graph_tile_runs(g);
     const TileDirectory dir = build_tile_directory(g);
     SyntheticQueryConfig qc;
     qc.count = a.queries;

@@ -216,8 +216,10 @@ const TileRunLayout64& graph_tile_runs(const WeightedGraph& graph) {
 DeviceGraphFingerprint fingerprint_device_graph_source32(const WeightedGraph& graph,
                                                          const TileRunLayout64& tile_runs) {
   // The tile runs are a function of the graph, so the graph's memo holds the
-  // value once runs equal to the graph's own have been seen.
-  if (tile_runs != graph_tile_runs(graph))
+  // value once runs equal to the graph's own have been seen.  The graph's own
+  // memoised runs need no element comparison (hundreds of MB on large graphs).
+  const TileRunLayout64& own = graph_tile_runs(graph);
+  if (&tile_runs != &own && tile_runs != own)
     throw std::invalid_argument{"device graph fingerprint: tile runs are not the graph's"};
   WeightedGraphMemo& memo = graph.memo();
   std::call_once(memo.device_fingerprint_once, [&] {

@@ -276,6 +276,8 @@ struct BatchedExpansionRunResult {
 // describe the batch it was given.  The runner must start every batch from
 // the queries' sources; a batch that stops for any reason other than clean
 // convergence classifies its lanes as engine_failure and never expands.
+// Graph checks are memoised with the graph; tile runs are compared with the
+// graph's element by element unless they are graph_tile_runs(graph) itself.
 [[nodiscard]] BatchedExpansionRunResult run_batched_expansion(
     const WeightedGraph& graph, const TileDirectory& directory, const TileRunLayout64& tile_runs,
     std::span<const RouteQuery> queries, const BatchedExpansionOptions& options,

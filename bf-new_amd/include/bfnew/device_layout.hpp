@@ -87,7 +87,8 @@ struct DeviceGraphFingerprint {
 // fingerprint_device_graph_layout32 of the layout built from the graph and
 // its tile runs (reference: src/device_layout.cpp:711-760).  Throws
 // std::invalid_argument when tile_runs are not the graph's.  The value is
-// kept with the graph (WeightedGraph::memo), so a graph pays for it once.
+// kept with the graph (WeightedGraph::memo), so a graph pays for it once;
+// passing graph_tile_runs(graph) also skips the element-wise comparison.
 [[nodiscard]] DeviceGraphFingerprint fingerprint_device_graph_source32(const WeightedGraph& graph,
                                                                        const TileRunLayout64& tile_runs);
 

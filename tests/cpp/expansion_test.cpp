@@ -1058,6 +1058,10 @@ void host_mode() {
       const auto rc2 = run_batched_expansion(copy, dir, runs, std::vector<RouteQuery>{q}, o, std::ref(copy_runner));
       CHECK(rc2.trace == r.trace && rc2.queries.front().final_query == oc.final_query &&
             rc2.queries.front().scheduled_expansions == oc.scheduled_expansions);
+      // The graph's memoised runs are taken without an element comparison.
+      const auto rc3 = run_batched_expansion(copy, dir, graph_tile_runs(copy), std::vector<RouteQuery>{q}, o,
+                                             std::ref(copy_runner));
+      CHECK(rc3.trace == r.trace && graph_tile_runs(copy) == runs && &graph_tile_runs(sg) == &graph_tile_runs(copy));
       WeightedGraph moved_from = sg;
       const WeightedGraph moved = std::move(moved_from);
       CHECK(dir.built_from(moved) && graph_region_index(moved).matches_graph(moved));
