@@ -182,6 +182,9 @@ HIP_SIGNATURES = {
     "bfn_device_query": ([C.c_void_p, C.POINTER(DeviceInfo)], C.c_int32),
     "bfn_device_synchronize": ([C.c_void_p], C.c_int32),
     "bfn_graph_upload": ([C.c_void_p, C.POINTER(GraphLayout32), C.POINTER(C.c_void_p)], C.c_int32),
+    "bfn_graph_validate": ([C.POINTER(GraphLayout32)], C.c_int32),
+    "bfn_batch_region_distances": ([C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64,
+                                    C.POINTER(C.c_uint64)], C.c_int32),
     "bfn_graph_free": ([C.c_void_p], C.c_int32),
     "bfn_graph_query": ([C.c_void_p, C.POINTER(GraphInfo)], C.c_int32),
     "bfn_graph_update_weights": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int32),
@@ -311,6 +314,35 @@ def _arr(ptr, n, dtype):
     if n == 0 or not ptr:
         return np.zeros(0, dtype=dtype)
     return np.ctypeslib.as_array(ptr, shape=(n,)).view(dtype)
+
+
+_LAYOUT_ARRAYS = [name for name, _ in GraphLayout32._fields_[6:]]
+
+
+def layout_from_arrays(arrays: dict) -> GraphLayout32:
+    """A GraphLayout32 over the numpy arrays of ``HostGraph.layout_arrays()``
+    (or a modified copy); the record keeps the arrays alive."""
+    L = GraphLayout32()
+    L.vertex_count, L.edge_count, L.tile_count = (arrays["vertex_count"], arrays["edge_count"],
+                                                  arrays["tile_count"])
+    L.csr_run_count = len(arrays["csr_run_destination_tiles"])
+    L.csc_run_count = len(arrays["csc_run_source_tiles"])
+    keep = []
+    for name in _LAYOUT_ARRAYS:
+        dtype = np.float32 if name.endswith("weights") else np.uint32
+        a = np.ascontiguousarray(arrays[name], dtype=dtype)
+        keep.append(a)
+        setattr(L, name, a.ctypes.data_as(_PF if dtype == np.float32 else _P32))
+    L._keep = keep
+    return L
+
+
+def validate_layout(layout: GraphLayout32):
+    """(status, message) of bfn_graph_validate: the host checks that
+    bfn_graph_upload runs before it touches the device."""
+    lib = hip_lib()
+    rc = lib.bfn_graph_validate(C.byref(layout))
+    return rc, (lib.bfn_last_error().decode() if rc else "")
 
 
 class HostGraph:
@@ -580,6 +612,15 @@ class Workspace:
         out = np.empty(self.vertex_count, np.float32)
         _check(hip_lib().bfn_batch_distances(self.h, query, _ptr(out)))
         return out
+
+    def region_distances(self, query: int) -> np.ndarray:
+        """Labels of the query's selected tiles, tile by tile (sorted tiles)."""
+        n = C.c_uint64()
+        _check(hip_lib().bfn_batch_region_distances(self.h, query, None, 0, C.byref(n)))
+        out = np.empty(max(n.value, 1), np.float32)
+        _check(hip_lib().bfn_batch_region_distances(self.h, query, _ptr(out), n.value,
+                                                    C.byref(n)))
+        return out[:n.value]
 
     def predecessors(self, query: int) -> np.ndarray:
         out = np.empty(self.vertex_count, np.uint32)

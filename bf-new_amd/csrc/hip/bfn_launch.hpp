@@ -74,6 +74,10 @@ hipError_t predecessors_launch(const DevGraph& g, const DevBatch& b, std::uint32
 hipError_t scatter_launch(const DevGraph& g, const DevBatch& b, std::uint32_t q,
                           std::uint32_t slots, std::uint32_t copy, std::uint32_t* out,
                           hipStream_t stream);
+hipError_t gather_region_launch(const DevGraph& g, const DevBatch& b, std::uint32_t q,
+                                std::uint32_t slots, std::uint32_t copy,
+                                const unsigned long long* offsets, std::uint32_t* out,
+                                hipStream_t stream);
 hipError_t finalize_launch(const DevGraph& g, const DevBatch& b,
                            const unsigned long long* relaxations, void* out, hipStream_t stream);
 hipError_t relaxation_count_launch(const DevGraph& g, const DevBatch& b,

@@ -162,14 +162,14 @@ struct bfn_workspace_s {
   // batch metadata in one device buffer + separately grown state
   DeviceBuffer meta, slot_in, slot_out, stamp, lists, pool, last_change, control, status, relax,
       scratch_v, paths_summary, paths_len, paths_off, frames, onpath, clean, arena_v, arena_l,
-      arena_e, occupancy;
+      arena_e, occupancy, region_off;
   PinnedBuffer staging, h_status, h_control, h_paths;
   // last batch
   bfn_graph graph = nullptr;
   DevBatch view{};
   std::uint32_t Q = 0, S = 0;
   std::uint64_t pool_words = 0, frame_words = 0;
-  std::vector<std::uint32_t> h_target_off, h_tile_off;
+  std::vector<std::uint32_t> h_target_off, h_tile_off, h_tiles;
   std::vector<std::uint32_t> h_engine;
   bfn_run_options options{};
   bfn_tuning tuning{};
@@ -251,6 +251,35 @@ bool offsets_valid(const std::uint32_t* off, std::uint32_t n, std::uint32_t tota
   return true;
 }
 
+// Tile runs of one orientation (device_layout.cpp:280-322): per row, the runs
+// cover the row's edges in order, are non-empty and maximal, name a tile in
+// range, and every edge of a run has its other endpoint in the run's tile.
+// The device derivation of the neighbour lists relies on all of it.
+const char* runs_invalid(std::uint32_t V, std::uint32_t T, const std::uint32_t* owner,
+                         const std::uint32_t* row_offsets, const std::uint32_t* endpoints,
+                         const std::uint32_t* row_runs, const std::uint32_t* run_edges,
+                         const std::uint32_t* run_tiles) {
+  for (std::uint32_t v = 0; v < V; ++v) {
+    std::uint32_t cursor = row_offsets[v];
+    std::uint32_t previous = kNone;
+    for (std::uint32_t r = row_runs[v]; r < row_runs[v + 1]; ++r) {
+      const std::uint32_t b = run_edges[r], e = run_edges[r + 1];
+      if (b != cursor) return "tile runs do not cover their row's edges in order";
+      if (e <= b) return "empty tile run";
+      if (e > row_offsets[v + 1]) return "tile run crosses its row";
+      const std::uint32_t tile = run_tiles[r];
+      if (tile >= T) return "tile run names a tile out of range";
+      if (tile == previous) return "tile runs are not maximal";
+      for (std::uint32_t x = b; x < e; ++x)
+        if (owner[endpoints[x]] != tile) return "an edge's endpoint is not in its run's tile";
+      previous = tile;
+      cursor = e;
+    }
+    if (cursor != row_offsets[v + 1]) return "tile runs do not cover their row's edges in order";
+  }
+  return nullptr;
+}
+
 template <typename T>
 hipError_t upload(DeviceBuffer& buf, const T* host, std::size_t count, hipStream_t stream) {
   hipError_t e = buf.reserve(sizeof(T) * std::max<std::size_t>(count, 1));
@@ -262,8 +291,13 @@ hipError_t upload(DeviceBuffer& buf, const T* host, std::size_t count, hipStream
 
 }  // namespace
 
-int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_graph* out) {
-  if (!device || !L || !out) return fail(BFN_E_INVALID_ARGUMENT, "null argument");
+namespace {
+
+// Checks of validate_device_graph_layout32 / validate_tile_run_layout
+// (device_layout.cpp:253-393) that the engine image depends on; writes the
+// largest tile size.
+int32_t validate_layout(const bfn_graph_layout32* L, std::uint32_t* largest_tile) {
+  if (!L) return fail(BFN_E_INVALID_ARGUMENT, "null layout");
   const std::uint32_t V = L->vertex_count, E = L->edge_count, T = L->tile_count;
   if (T == 0) return fail(BFN_E_INVALID_ARGUMENT, "graph has no tiles");
   if (!L->owner_tiles && V) return fail(BFN_E_INVALID_ARGUMENT, "missing owner tiles");
@@ -286,9 +320,41 @@ int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_gra
   }
   if (max_v > kMaxTileVertices)
     return fail(BFN_E_NOT_REPRESENTABLE, "a tile holds more than 4096 vertices");
+  if (E && (!L->csr_destinations || !L->csr_weights || !L->csc_sources || !L->csc_weights ||
+            !L->csc_edge_ids))
+    return fail(BFN_E_INVALID_ARGUMENT, "missing edge arrays");
+  for (std::uint32_t e = 0; e < E; ++e)
+    if (!(L->csr_weights[e] >= 0.0f) || !(L->csc_weights[e] >= 0.0f))
+      return fail(BFN_E_INVALID_ARGUMENT, "edge weights must be nonnegative");
   for (std::uint32_t e = 0; e < E; ++e)
     if (L->csr_destinations[e] >= V || L->csc_sources[e] >= V || L->csc_edge_ids[e] >= E)
       return fail(BFN_E_INVALID_ARGUMENT, "edge endpoint or edge id out of range");
+  if (!L->csr_run_destination_tiles || !L->csc_run_source_tiles)
+    return fail(BFN_E_INVALID_ARGUMENT, "missing tile-run tiles");
+  if (const char* why = runs_invalid(V, T, L->owner_tiles, L->csr_row_offsets, L->csr_destinations,
+                                     L->csr_row_run_offsets, L->csr_run_edge_offsets,
+                                     L->csr_run_destination_tiles))
+    return fail(BFN_E_INVALID_ARGUMENT, std::string("CSR: ") + why);
+  if (const char* why = runs_invalid(V, T, L->owner_tiles, L->csc_column_offsets, L->csc_sources,
+                                     L->csc_column_run_offsets, L->csc_run_edge_offsets,
+                                     L->csc_run_source_tiles))
+    return fail(BFN_E_INVALID_ARGUMENT, std::string("CSC: ") + why);
+  *largest_tile = max_v;
+  return BFN_OK;
+}
+
+}  // namespace
+
+int32_t bfn_graph_validate(const bfn_graph_layout32* layout) {
+  std::uint32_t max_v = 0;
+  return validate_layout(layout, &max_v);
+}
+
+int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_graph* out) {
+  if (!device || !L || !out) return fail(BFN_E_INVALID_ARGUMENT, "null argument");
+  std::uint32_t max_v = 0;
+  if (int32_t rc = validate_layout(L, &max_v)) return rc;
+  const std::uint32_t V = L->vertex_count, E = L->edge_count, T = L->tile_count;
 
   BFN_HIP(hipSetDevice(device->ordinal));
   auto* g = new bfn_graph_s;
@@ -908,6 +974,35 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
   if (ws->budget && need > ws->budget)
     return fail(BFN_E_OUT_OF_MEMORY, "batch needs " + std::to_string(need) +
                                          " bytes of workspace, budget is " + std::to_string(ws->budget));
+
+  // Launch shape, checked before the workspace changes: a refused batch
+  // leaves the workspace without a batch (status, results and rerun then
+  // report BFN_E_STATE), never a mix of the old and the new one.
+  const DevGraph& gv = graph->view;
+  const std::uint32_t max_v = gv.max_tile_vertices;
+  const std::size_t lds_limit = ws->device->props.sharedMemPerBlock;
+  // Push engines stage their boundary proposals after the snapshot when the
+  // LDS holds them; otherwise they push with direct global atomics.
+  const bool proposals = p.any_push && t.boundary_mode == 0 &&
+                         lds_words(max_v, gv.max_tile_vertices + gv.max_out_stage_floats) * 4ull <=
+                             lds_limit;
+  const std::uint32_t work_words =
+      std::max({p.any_jacobi ? gv.max_stage_floats : 0u, gv.max_tile_vertices,
+                proposals ? gv.max_tile_vertices + gv.max_out_stage_floats : 0u});
+  const std::size_t lds_bytes = 4ull * lds_words(max_v, work_words);
+  if (lds_bytes > lds_limit)
+    return fail(BFN_E_NOT_REPRESENTABLE, "tile neighbourhood exceeds the LDS budget (" +
+                                             std::to_string(lds_bytes) + " > " +
+                                             std::to_string(lds_limit) + " bytes)");
+  int per_cu = 0;
+  BFN_HIP(relax_occupancy(o->block_size, o->control_mode == 0, o->instrumentation != 0,
+                          t.min_waves, lds_bytes, &per_cu));
+  if (per_cu <= 0) return fail(BFN_E_NOT_REPRESENTABLE, "relaxation kernel cannot be resident");
+  const std::uint32_t cus = static_cast<std::uint32_t>(ws->device->props.multiProcessorCount);
+  std::uint32_t blocks = static_cast<std::uint32_t>(per_cu) * cus;
+  if (o->grid_policy == 1) blocks = std::min(blocks, o->blocks_per_wgp * cus);
+
+  ws->submitted = ws->completed = ws->have_paths = ws->paths_counted = false;
   BFN_HIP(ws->staging.reserve(meta_bytes));
   char* h = ws->staging.as<char>();
   std::memcpy(h + seg[0], d->tile_offsets, sizes[0]);
@@ -948,36 +1043,16 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
   ws->tuning = t;
   ws->h_target_off.assign(d->target_offsets, d->target_offsets + Q + 1);
   ws->h_tile_off.assign(d->tile_offsets, d->tile_offsets + Q + 1);
+  ws->h_tiles.assign(d->selected_tiles, d->selected_tiles + S);
   ws->h_engine = p.engine;
   ws->launches = ws->host_checks = ws->host_syncs = ws->controller_copies = 0;
   bind_view(ws, p, seg, o, &t);
   ws->frame_base_ptr = reinterpret_cast<const unsigned long long*>(ws->meta.as<char>() + seg[13]);
-
-  // Launch shape.
-  const DevGraph& gv = graph->view;
-  ws->block = o->block_size;
-  ws->max_v = gv.max_tile_vertices;
-  const std::size_t lds_limit = ws->device->props.sharedMemPerBlock;
-  // Push engines stage their boundary proposals after the snapshot when the
-  // LDS holds them; otherwise they push with direct global atomics.
-  const bool proposals = p.any_push && t.boundary_mode == 0 &&
-                         lds_words(ws->max_v, gv.max_tile_vertices + gv.max_out_stage_floats) * 4ull <=
-                             lds_limit;
   if (!proposals) ws->view.boundary_mode = 1;
-  ws->work_words = std::max({p.any_jacobi ? gv.max_stage_floats : 0u, gv.max_tile_vertices,
-                             proposals ? gv.max_tile_vertices + gv.max_out_stage_floats : 0u});
-  ws->lds_bytes = 4ull * lds_words(ws->max_v, ws->work_words);
-  if (ws->lds_bytes > lds_limit)
-    return fail(BFN_E_NOT_REPRESENTABLE, "tile neighbourhood exceeds the LDS budget (" +
-                                             std::to_string(ws->lds_bytes) + " > " +
-                                             std::to_string(lds_limit) + " bytes)");
-  int per_cu = 0;
-  BFN_HIP(relax_occupancy(ws->block, o->control_mode == 0, o->instrumentation != 0, t.min_waves,
-                          ws->lds_bytes, &per_cu));
-  if (per_cu <= 0) return fail(BFN_E_NOT_REPRESENTABLE, "relaxation kernel cannot be resident");
-  const std::uint32_t cus = static_cast<std::uint32_t>(ws->device->props.multiProcessorCount);
-  std::uint32_t blocks = static_cast<std::uint32_t>(per_cu) * cus;
-  if (o->grid_policy == 1) blocks = std::min(blocks, o->blocks_per_wgp * cus);
+  ws->block = o->block_size;
+  ws->max_v = max_v;
+  ws->work_words = work_words;
+  ws->lds_bytes = lds_bytes;
   ws->grid = std::max(1u, blocks);
   ws->submitted = true;
   return run_resident_batch(ws);
@@ -1078,6 +1153,36 @@ int32_t bfn_batch_distances(bfn_workspace ws, uint32_t query, float* out) {
   BFN_HIP(scatter_launch(ws->graph->view, ws->view, query, slots, copy,
                          ws->scratch_v.as<std::uint32_t>(), s));
   BFN_HIP(hipMemcpyAsync(out, ws->scratch_v.ptr, 4ull * ws->graph->V, hipMemcpyDeviceToHost, s));
+  BFN_HIP(hipStreamSynchronize(s));
+  return BFN_OK;
+}
+
+int32_t bfn_batch_region_distances(bfn_workspace ws, uint32_t query, float* out,
+                                   uint64_t capacity, uint64_t* count) {
+  if (!ws || !ws->completed) return fail(BFN_E_STATE, "batch has not completed");
+  if (query >= ws->Q) return fail(BFN_E_INVALID_ARGUMENT, "query index out of range");
+  if (!count) return fail(BFN_E_INVALID_ARGUMENT, "null count");
+  const std::uint32_t p0 = ws->h_tile_off[query], p1 = ws->h_tile_off[query + 1];
+  const auto& toff = ws->graph->h_tile_off;
+  std::vector<unsigned long long> offsets(p1 - p0);
+  std::uint64_t n = 0;
+  for (std::uint32_t i = p0; i < p1; ++i) {
+    offsets[i - p0] = n;
+    n += toff[ws->h_tiles[i] + 1] - toff[ws->h_tiles[i]];
+  }
+  *count = n;
+  if (!out) return BFN_OK;
+  if (capacity < n) return fail(BFN_E_INVALID_ARGUMENT, "output capacity too small");
+  if (int32_t rc = ensure_scratch_v(ws)) return rc;
+  BFN_HIP(ws->region_off.reserve(8ull * std::max<std::size_t>(offsets.size(), 1)));
+  hipStream_t s = ws->stream;
+  const DevControl* c = ws->h_control.as<DevControl>();
+  BFN_HIP(hipMemcpyAsync(ws->region_off.ptr, offsets.data(), 8ull * offsets.size(),
+                         hipMemcpyHostToDevice, s));
+  BFN_HIP(gather_region_launch(ws->graph->view, ws->view, query, p1 - p0,
+                               c->rounds_executed & 1u, ws->region_off.as<unsigned long long>(),
+                               ws->scratch_v.as<std::uint32_t>(), s));
+  if (n) BFN_HIP(hipMemcpyAsync(out, ws->scratch_v.ptr, 4ull * n, hipMemcpyDeviceToHost, s));
   BFN_HIP(hipStreamSynchronize(s));
   return BFN_OK;
 }

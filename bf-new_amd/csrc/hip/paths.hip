@@ -362,6 +362,33 @@ hipError_t scatter_launch(const DevGraph& g, const DevBatch& b, std::uint32_t q,
   return hipGetLastError();
 }
 
+// Copies query q's final labels tile by tile, in the order of its sorted
+// selected tiles, to out[offsets[i] ...] (i = position within the query).
+__global__ void gather_region_kernel(DevGraph g, DevBatch b, std::uint32_t q,
+                                     std::uint32_t jacobi_copy, const unsigned long long* offsets,
+                                     std::uint32_t* out) {
+  const std::uint32_t i = blockIdx.x;
+  const std::uint32_t pos = b.tile_begin[q] + i;
+  if (pos >= b.tile_end[q]) return;
+  const std::uint32_t s = slot_at(b, pos);
+  const std::uint32_t T = b.tiles[s];
+  const std::uint32_t npt = g.tile_vertex_offsets[T + 1] - g.tile_vertex_offsets[T];
+  const std::uint32_t* src =
+      b.dist + b.slot_dist[s] + static_cast<unsigned long long>(column_of(b, q, jacobi_copy)) * npt;
+  std::uint32_t* dst = out + offsets[i];
+  for (std::uint32_t k = threadIdx.x; k < npt; k += blockDim.x) dst[k] = src[k];
+}
+
+hipError_t gather_region_launch(const DevGraph& g, const DevBatch& b, std::uint32_t q,
+                                std::uint32_t slots, std::uint32_t copy,
+                                const unsigned long long* offsets, std::uint32_t* out,
+                                hipStream_t stream) {
+  if (slots)
+    hipLaunchKernelGGL(gather_region_kernel, dim3(slots), dim3(256), 0, stream, g, b, q, copy,
+                       offsets, out);
+  return hipGetLastError();
+}
+
 hipError_t finalize_launch(const DevGraph& g, const DevBatch& b,
                            const unsigned long long* relaxations, void* out, hipStream_t stream) {
   hipLaunchKernelGGL(finalize_kernel, dim3((b.query_count + 127) / 128), dim3(128), 0, stream, g,

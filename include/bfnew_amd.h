@@ -224,6 +224,14 @@ int32_t bfn_device_synchronize(bfn_device device);
  * engine image (tile neighbour lists, packed tile-local CSC/CSR) on device. */
 int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32 *layout,
                          bfn_graph *out);
+/* The host checks bfn_graph_upload runs before it touches the device
+ * (validate_device_graph_layout32 / validate_tile_run_layout,
+ * device_layout.cpp:253-393): offsets, owner tiles, endpoints, nonnegative
+ * weights and the tile runs of both orientations (in range, non-empty,
+ * maximal, covering each row in order, each edge's endpoint in its run's
+ * tile).  BFN_E_INVALID_ARGUMENT or BFN_E_NOT_REPRESENTABLE on failure;
+ * needs no GPU. */
+int32_t bfn_graph_validate(const bfn_graph_layout32 *layout);
 int32_t bfn_graph_free(bfn_graph graph);
 int32_t bfn_graph_query(bfn_graph graph, bfn_graph_info *out);
 /* Replaces every edge weight; both orientations in the layout's orders. */
@@ -265,6 +273,13 @@ int32_t bfn_batch_status(bfn_workspace workspace, bfn_run_status *status,
 /* run_with_distances: one query's labels scattered to global ids (+inf
  * outside its selected tiles). */
 int32_t bfn_batch_distances(bfn_workspace workspace, uint32_t query, float *out);
+/* One query's final labels over its selected tiles only, tile by tile in
+ * the order of its sorted selected tiles (each tile's vertices in id order):
+ * the compact form of run_with_distances that DeviceCompactDistanceMatrix
+ * (include/bfnew/hip/compact_path_results.hpp:23-31) keeps per lane.  With
+ * out == NULL only *count (the label count) is written. */
+int32_t bfn_batch_region_distances(bfn_workspace workspace, uint32_t query, float *out,
+                                   uint64_t capacity, uint64_t *count);
 /* Smallest-stable-EdgeId tight admitted predecessor per vertex (0xFFFFFFFF
  * for sources, unreached and unselected vertices). */
 int32_t bfn_batch_predecessors(bfn_workspace workspace, uint32_t query, uint32_t *out);

@@ -675,3 +675,103 @@ int orc_reconstruct(uint32_t V, const uint64_t *column_offsets, const uint32_t *
   free(rev_edges);
   return rc;
 }
+
+/* ---------------------------------------------------------------------------
+ * Exactness certificate of one query's labels over its selected tiles
+ * (no reference counterpart: it checks the result that
+ * reference_dijkstra.cpp:32-63 defines, without running Dijkstra).
+ *
+ * labels[] holds the query's labels tile by tile in the order of tiles[]
+ * (sorted, unique): tile tiles[i] contributes tile_vertex_offsets[t+1] -
+ * tile_vertex_offsets[t] labels.  Admitted edges are those whose two
+ * endpoints lie in selected tiles (query.cpp:286-362).  The labels are the
+ * shortest-path fixed point exactly when
+ *   (1) every source is +0 and every label is +0, finite positive or +inf;
+ *   (2) no admitted edge lowers a label: fl(d[u] + w) >= d[v];
+ *   (3) every finite non-source label d[v] has a tight admitted in-edge,
+ *       fl(d[u] + w) == d[v], with d[u] < d[v].
+ * (3) with the strict decrease makes every finite label the left-to-right
+ * fp32 sum of an admitted path from a source; (2) with the monotonicity of
+ * fl(x + w) in x makes it no larger than any admitted path's sum; an
+ * unreached +inf label next to a finite one fails (2).  Absorption
+ * (fl(d[u] + w) == d[u]) can never count as tight, so zero or tiny weights
+ * cannot certify a false label.  Returns 0 or the failed rule (1-3) and, in
+ * *where, the compact index of the offending label.
+ * ------------------------------------------------------------------------- */
+int orc_certify_region(uint32_t tile_count, const uint32_t *tile_vertex_offsets,
+                       const uint32_t *column_offsets, const uint32_t *sources_csc,
+                       const float *in_weights, const uint32_t *owner_tiles,
+                       uint32_t n_tiles, const uint32_t *tiles, const float *labels,
+                       uint32_t source_count, const uint32_t *sources, uint64_t *where) {
+  int64_t *base = malloc(sizeof(int64_t) * (tile_count ? tile_count : 1));
+  if (!base) return ORC_NOMEM;
+  for (uint32_t t = 0; t < tile_count; ++t) base[t] = -1;
+  uint64_t n = 0;
+  for (uint32_t i = 0; i < n_tiles; ++i) {
+    const uint32_t t = tiles[i];
+    if (t >= tile_count || (i && tiles[i - 1] >= t)) {
+      free(base);
+      return ORC_INVALID;
+    }
+    base[t] = (int64_t)n;
+    n += tile_vertex_offsets[t + 1] - tile_vertex_offsets[t];
+  }
+  int rule = 0;
+  uint64_t at = 0;
+  uint8_t *is_source = calloc(n ? n : 1, 1);
+  if (!is_source) {
+    free(base);
+    return ORC_NOMEM;
+  }
+  for (uint32_t i = 0; i < source_count && !rule; ++i) {
+    const uint32_t s = sources[i];
+    const uint32_t t = owner_tiles[s];
+    if (base[t] < 0) {
+      rule = 1;
+      break;
+    }
+    const uint64_t k = (uint64_t)base[t] + (s - tile_vertex_offsets[t]);
+    is_source[k] = 1;
+    if (float_bits(labels[k]) != 0u) {
+      rule = 1;
+      at = k;
+    }
+  }
+  for (uint64_t k = 0; k < n && !rule; ++k) {
+    const uint32_t b = float_bits(labels[k]);
+    if (b > 0x7F800000u) {
+      rule = 1;
+      at = k;
+    }
+  }
+  for (uint32_t i = 0; i < n_tiles && !rule; ++i) {
+    const uint32_t t = tiles[i];
+    const uint32_t v0 = tile_vertex_offsets[t], v1 = tile_vertex_offsets[t + 1];
+    for (uint32_t v = v0; v < v1 && !rule; ++v) {
+      const uint64_t kv = (uint64_t)base[t] + (v - v0);
+      const float dv = labels[kv];
+      int tight = 0;
+      for (uint32_t e = column_offsets[v]; e < column_offsets[v + 1]; ++e) {
+        const uint32_t u = sources_csc[e];
+        const uint32_t tu = owner_tiles[u];
+        if (base[tu] < 0) continue;
+        const float du = labels[(uint64_t)base[tu] + (u - tile_vertex_offsets[tu])];
+        const float cand = du + in_weights[e];
+        if (cand < dv) {
+          rule = 2;
+          at = kv;
+          break;
+        }
+        if (cand == dv && du < dv) tight = 1;
+      }
+      if (!rule && dv != INFINITY && !is_source[kv] && !tight) {
+        rule = 3;
+        at = kv;
+      }
+    }
+  }
+  free(is_source);
+  free(base);
+  if (where) *where = at;
+  return rule;
+}

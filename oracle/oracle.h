@@ -27,6 +27,12 @@
  *   orc_reconstruct    -- literal paths tests/sssp_test.cpp:174-227 and
  *                         tests/compact_paths_test.cpp:114-205, and the
  *                         reference reconstruct_path_from_distances (_ref).
+ *   orc_certify_region -- accepts the oracle's own Dijkstra labels on every
+ *                         fixture and synthetic case, and rejects each of a
+ *                         set of planted mistakes (a label one ulp low or
+ *                         high, a spurious finite label on an unreachable
+ *                         cycle, an absorbed zero-weight edge, a nonzero
+ *                         source) in tests/test_oracle.py.
  */
 #ifndef BFNEW_ORACLE_H
 #define BFNEW_ORACLE_H
@@ -160,6 +166,17 @@ int orc_reconstruct(uint32_t vertex_count, const uint64_t *column_offsets,
                     const uint32_t *sources, uint32_t source_count,
                     uint32_t target, uint32_t capacity, uint32_t *path_vertices,
                     uint64_t *path_edge_ids, uint32_t *path_length);
+
+/* Size-independent exactness certificate of one query's labels, given tile
+ * by tile over its sorted selected tiles (see oracle.c for the argument).
+ * Returns 0 when the labels are the exact fp32 shortest-path fixed point of
+ * the admitted subgraph, else the failed rule 1-3 with the compact index of
+ * the offending label in *where. */
+int orc_certify_region(uint32_t tile_count, const uint32_t *tile_vertex_offsets,
+                       const uint32_t *column_offsets, const uint32_t *sources_csc,
+                       const float *in_weights, const uint32_t *owner_tiles,
+                       uint32_t n_tiles, const uint32_t *tiles, const float *labels,
+                       uint32_t source_count, const uint32_t *sources, uint64_t *where);
 
 #ifdef __cplusplus
 }

@@ -84,6 +84,8 @@ def lib():
         _lib.orc_predecessors.argtypes = [C.c_uint32, P, P, P, P, P, P, P, P, C.c_uint32, P, P]
         _lib.orc_reconstruct.argtypes = [C.c_uint32, P, P, P, P, P, P, P, P, C.c_uint32,
                                          C.c_uint32, C.c_uint32, P, P, P]
+        _lib.orc_certify_region.argtypes = [C.c_uint32, P, P, P, P, P, C.c_uint32, P, P,
+                                            C.c_uint32, P, C.POINTER(C.c_uint64)]
     return _lib
 
 
@@ -263,6 +265,41 @@ def reconstruct(V, column_offsets, sources_csc, in_weights, in_edge_ids, owner_t
         raise ValueError(f"oracle reconstruct failed: {rc}")
     L = n.value
     return "complete", pv[:L + 1].copy(), pe[:L].copy()
+
+
+CERTIFY_RULES = {0: "", 1: "a source label is not +0 or a label is not a nonnegative float",
+                 2: "an admitted edge still lowers a label (not a fixed point)",
+                 3: "a finite label has no tight admitted in-edge from a smaller label"}
+
+
+def region_vertices(tile_vertex_offsets, tiles):
+    """Global vertex ids of the selected tiles, in the compact (tile by tile)
+    order of ``certify_region``'s labels."""
+    tvo = np.asarray(tile_vertex_offsets, np.int64)
+    t = np.asarray(tiles, np.int64)
+    return np.concatenate([np.arange(tvo[x], tvo[x + 1], dtype=np.int64) for x in t]) \
+        if len(t) else np.zeros(0, np.int64)
+
+
+def certify_region(layout, tiles, labels, sources):
+    """orc_certify_region over a product layout (``HostGraph.layout_arrays()``):
+    labels tile by tile over the sorted selected tiles.  Returns (message,
+    compact index); message '' means certified exact.  Releases the GIL, so
+    queries can be certified on several threads."""
+    tvo = np.ascontiguousarray(layout["tile_vertex_offsets"], np.uint32)
+    col = np.ascontiguousarray(layout["csc_column_offsets"], np.uint32)
+    srcs = np.ascontiguousarray(layout["csc_sources"], np.uint32)
+    w = np.ascontiguousarray(layout["csc_weights"], np.float32)
+    own = np.ascontiguousarray(layout["owner_tiles"], np.uint32)
+    t = np.ascontiguousarray(tiles, np.uint32)
+    lab = np.ascontiguousarray(labels, np.float32)
+    s = np.ascontiguousarray(sources, np.uint32)
+    where = C.c_uint64(0)
+    rc = lib().orc_certify_region(layout["tile_count"], _p(tvo), _p(col), _p(srcs), _p(w), _p(own),
+                                  len(t), _p(t), _p(lab), len(s), _p(s), C.byref(where))
+    if rc < 0:
+        raise ValueError(f"oracle certificate failed: {rc}")
+    return CERTIFY_RULES[rc], where.value
 
 
 def graph_sssp(g: Graph, sources, tile_selected=None):

@@ -105,3 +105,46 @@ def certify_sssp(layout, sources, tiles, dist):
     if np.any(w[adm] <= 0):
         return "certificate needs strictly positive admitted weights"
     return ""
+
+
+def certify_batch(layout, ws, sources, tiles, threads=16):
+    """Certifies every query of a completed workspace with the oracle's region
+    certificate: labels come off the device one query at a time (the
+    workspace is single-stream), certificates run on `threads` host threads.
+    Returns [(query, message)] of the failures."""
+    from concurrent.futures import ThreadPoolExecutor
+    failures = []
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        futures = []
+        for q in range(len(tiles)):
+            labels = ws.region_distances(q)
+            futures.append((q, pool.submit(oracle.certify_region, layout, tiles[q], labels,
+                                           sources[q])))
+        for q, f in futures:
+            msg, _ = f.result()
+            if msg:
+                failures.append((q, msg))
+    return failures
+
+
+def stratified(sizes, per_stratum=2):
+    """Indices of the smallest, median and largest entries of `sizes`."""
+    order = sorted(range(len(sizes)), key=lambda i: (sizes[i], i))
+    n = len(order)
+    mid = n // 2
+    picked = order[:per_stratum] + order[mid:mid + per_stratum] + order[n - per_stratum:]
+    return sorted(set(picked))
+
+
+def csr_view(layout):
+    """The CSR part of a product layout in the form the oracle's Dijkstra takes."""
+    return dict(vertex_count=layout["vertex_count"], tile_count=layout["tile_count"],
+                row_offsets=layout["csr_row_offsets"].astype(np.uint64),
+                destinations=layout["csr_destinations"], weights=layout["csr_weights"],
+                owner_tiles=layout["owner_tiles"])
+
+
+def oracle_region(view, layout, source, tiles):
+    """Oracle Dijkstra labels of one query, in the region (tile by tile) order."""
+    d, _ = oracle_query(view, [source], tiles)
+    return d[oracle.region_vertices(layout["tile_vertex_offsets"], tiles)]

@@ -61,3 +61,41 @@ def test_null_arguments_are_rejected():
     assert lib.bfn_graph_upload(None, None, None) == 1
     assert lib.bfn_batch_wait(None) == 5
     assert b"no batch" in lib.bfn_last_error()
+
+
+def _small_layout():
+    hg = bfn.HostGraph.synthetic(4, 4, 16, 9)
+    return hg.layout_arrays()
+
+
+def test_validate_accepts_builder_layout():
+    rc, msg = bfn.validate_layout(bfn.layout_from_arrays(_small_layout()))
+    assert rc == 0, msg
+
+
+def test_validate_rejects_bad_tile_runs():
+    """The reference's tile-run checks (device_layout.cpp:280-322): a run tile
+    out of range, a run naming a tile that is not its edges' endpoint tile,
+    an empty run and negative weights are refused before any device call."""
+    base = _small_layout()
+    T = base["tile_count"]
+    cases = []
+    a = {k: (v.copy() if hasattr(v, "copy") else v) for k, v in base.items()}
+    a["csr_run_destination_tiles"][3] = T + 5
+    cases.append((a, "out of range"))
+    a = {k: (v.copy() if hasattr(v, "copy") else v) for k, v in base.items()}
+    r = a["csc_run_source_tiles"]
+    r[0] = (r[0] + 1) % T
+    cases.append((a, "run's tile"))
+    a = {k: (v.copy() if hasattr(v, "copy") else v) for k, v in base.items()}
+    # collapse the first CSR run of vertex 0 onto its start: it becomes empty
+    ro, re_ = a["csr_row_run_offsets"], a["csr_run_edge_offsets"]
+    assert ro[1] - ro[0] >= 2
+    re_[1] = re_[0]
+    cases.append((a, "run"))
+    a = {k: (v.copy() if hasattr(v, "copy") else v) for k, v in base.items()}
+    a["csc_weights"][7] = -1.0
+    cases.append((a, "nonnegative"))
+    for arrays, needle in cases:
+        rc, msg = bfn.validate_layout(bfn.layout_from_arrays(arrays))
+        assert rc == 1 and needle in msg, (rc, msg)

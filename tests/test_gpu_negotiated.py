@@ -57,3 +57,78 @@ def test_negotiated_rounds_bit_exact(device, engine):
         dg.update_node_costs_from_workspace(ws)
     ws.close()
     dg.close()
+
+
+@pytest.mark.timeout(900)
+def test_c5_negotiated_rounds_at_size(device):
+    """Config 5 as configured: 256x256x512 graph (33.5 M vertices, 335 M
+    edges), 1024 queries with box sides 4-64, four rounds of route -> compact
+    paths -> occupancy -> node-cost update, batched in one persistent
+    cooperative launch per round.  Every round: all 1024 queries certified
+    exact by the oracle's region certificate under weights recomputed on the
+    host from the host's own occupancy count, oracle Dijkstra and path
+    reconstruction bit-exact on a stratified sample that includes the
+    largest boxes, and the device occupancy equal to the host count over the
+    returned paths."""
+    from helpers import certify_batch, oracle_region, stratified
+    print("c5: generating 256x256x512", flush=True)
+    hg = bfn.HostGraph.synthetic(256, 256, 512, 5, host_graph=False)
+    lay = hg.layout_arrays()
+    V, T = lay["vertex_count"], lay["tile_count"]
+    dg = bfn.DeviceGraph(device, hg.layout())
+    # The generator gives every edge u->v the node cost of v: read it off
+    # the CSC (vertices without in-edges keep 1.0; no weight depends on it).
+    indeg = np.diff(lay["csc_column_offsets"]).astype(np.int64)
+    col_vertex = np.repeat(np.arange(V, dtype=np.uint32), indeg)
+    base = np.ones(V, np.float32)
+    has_in = indeg > 0
+    base[has_in] = lay["csc_weights"][lay["csc_column_offsets"][:-1][has_in]]
+    assert np.array_equal(lay["csc_weights"], base[col_vertex])
+    qs = hg.synthetic_queries(1024, seed=55, side_min=4, side_max=64)
+    tiles = [hg.query_tiles(q) for q in qs]
+    sources = [[q.source] for q in qs]
+    batch = bfn.Batch(sources, [[q.target] for q in qs], tiles)
+    sample = stratified([len(t) for t in tiles])
+    dg.update_node_costs(base)
+    ws = bfn.Workspace(device)
+    occ = np.zeros(V, np.int32)
+    for rnd in range(4):
+        print(f"c5 round {rnd}: relax", flush=True)
+        ws.submit(dg, batch, bfn.run_options(engine=bfn.FRONTIER_PUSH, control_mode=bfn.PERSISTENT),
+                  bfn.tuning(tile_sweeps=0, boundary_mode=1, items_per_claim=2))
+        assert ws.status()[0].converged == 1, rnd
+        cost = base * (np.float32(1.0) + np.float32(0.5) * occ.astype(np.float32))
+        rl = dict(lay, csc_weights=cost[col_vertex])
+        print(f"c5 round {rnd}: certify 1024 queries", flush=True)
+        failures = certify_batch(rl, ws, sources, tiles)
+        assert not failures, (rnd, failures[:5])
+        view = dict(vertex_count=V, tile_count=T,
+                    row_offsets=lay["csr_row_offsets"].astype(np.uint64),
+                    destinations=lay["csr_destinations"],
+                    weights=cost[lay["csr_destinations"]], owner_tiles=lay["owner_tiles"])
+        paths = ws.paths()
+        print(f"c5 round {rnd}: oracle sample {sample}", flush=True)
+        for i in sample:
+            ref = oracle_region(view, lay, qs[i].source, tiles[i])
+            assert np.array_equal(ws.region_distances(i).view(np.uint32), ref.view(np.uint32)), \
+                (rnd, i)
+            sel = np.zeros(T, np.uint8)
+            sel[tiles[i]] = 1
+            full = np.full(V, np.inf, np.float32)
+            full[oracle.region_vertices(lay["tile_vertex_offsets"], tiles[i])] = ref
+            status, ov, oe = oracle.reconstruct(
+                V, lay["csc_column_offsets"].astype(np.uint64), lay["csc_sources"],
+                rl["csc_weights"], lay["csc_edge_ids"].astype(np.uint64), lay["owner_tiles"], sel,
+                full, [qs[i].source], qs[i].target)
+            assert status == "complete" and paths[i]["reconstruction"] == 0, (rnd, i)
+            assert list(paths[i]["vertices"]) == list(ov), (rnd, i)
+            assert list(paths[i]["edge_ids"]) == list(oe), (rnd, i)
+        host_occ = np.zeros(V, np.int32)
+        for p in paths:
+            if p["reconstruction"] == 0:
+                np.add.at(host_occ, np.asarray(p["vertices"], np.int64), 1)
+        assert np.array_equal(ws.occupancy(), host_occ), rnd
+        occ = host_occ
+        dg.update_node_costs_from_workspace(ws)
+    ws.close()
+    dg.close()

@@ -8,7 +8,8 @@ import pytest
 
 import bfnew_amd as bfn
 import oracle
-from helpers import bits, certify_sssp, fbits, fixture_arrays, oracle_query, tile_mask
+from helpers import (bits, certify_batch, certify_sssp, csr_view, fbits, fixture_arrays,
+                     oracle_query, oracle_region, stratified, tile_mask)
 
 pytestmark = pytest.mark.gpu
 
@@ -201,10 +202,15 @@ def c2_batch(hg, count, seed=2):
     return qs, bfn.Batch([[q.source] for q in qs], [[q.target] for q in qs], tiles)
 
 
+@pytest.mark.timeout(600)
 def test_c2_batched_and_plain_bit_exact(device):
+    """Config 2 (64x64x256, 64 queries, sides 4-32): the whole batch under
+    every engine and boundary/occupancy variant, and plain mode -- each query
+    alone in its own persistent launch, for each of the three engines --
+    bit-exact against oracle Dijkstra for all 64 queries."""
     hg, dg = synthetic(device, 64, 64, 256, 2)
     view = hg.view()
-    qs, batch = c2_batch(hg, 24)
+    qs, batch = c2_batch(hg, 64)
     refs = [oracle_query(view, [q.source], batch.tiles[i])[0] for i, q in enumerate(qs)]
     for eng, knobs in ((0, {"sweeps": 0}), (0, {"sweeps": 1}), (1, {"boundary_mode": 0}),
                        (1, {"boundary_mode": 1}), (2, {"boundary_mode": 0}),
@@ -214,7 +220,18 @@ def test_c2_batched_and_plain_bit_exact(device):
         for i in range(len(qs)):
             assert np.array_equal(bits(ws.distances(i)), bits(refs[i])), (eng, knobs, i)
         ws.close()
-    # plain mode: one query per launch
+    # plain mode: one query per launch, one workspace per engine reused by
+    # every query (each submit rebinds it)
+    for eng in ENGINES:
+        ws = bfn.Workspace(device)
+        for i in range(len(qs)):
+            one = bfn.Batch([[qs[i].source]], [[qs[i].target]], [batch.tiles[i]])
+            ws.submit(dg, one, bfn.run_options(engine=eng, control_mode=bfn.PERSISTENT),
+                      bfn.tuning(tile_sweeps=0, boundary_mode=1))
+            assert ws.status()[0].converged == 1, (eng, i)
+            assert np.array_equal(bits(ws.distances(0)), bits(refs[i])), (eng, i)
+        ws.close()
+    # the reference's point-Jacobi round semantics, host-polled per round
     for i in range(4):
         one = bfn.Batch([[qs[i].source]], [[qs[i].target]], [batch.tiles[i]])
         for eng in ENGINES:
@@ -266,34 +283,72 @@ def test_relaxation_counts_match_oracle(device):
         assert per[i].relaxations == relax
 
 
+@pytest.mark.timeout(600)
 def test_c3_batch_certificate(device):
-    """Config 3 at full size: 256 queries in one batched persistent launch.
-    Exactness for all queries through the size-independent certificate, and
-    bit-exact agreement with the oracle on a sample."""
+    """Config 3 at full size (128x128x512, 256 queries, sides 4-64, 1%
+    unbounded): every query of three runs -- block Jacobi in one persistent
+    cooperative launch, the tuned frontier push replayed through a hipGraph,
+    and dense chaotic push -- certified exact by the oracle's region
+    certificate, and a stratified sample (smallest, median and largest
+    boxes, plus the unbounded queries) bit-exact against oracle Dijkstra."""
     hg, dg = synthetic(device, 128, 128, 512, 3, host_graph=False)
     lay = hg.layout_arrays()
+    view = csr_view(lay)
     qs = hg.synthetic_queries(256, seed=33, side_min=4, side_max=64, unbounded_fraction=0.01)
     tiles = [hg.query_tiles(q) for q in qs]
-    batch = bfn.Batch([[q.source] for q in qs], [[q.target] for q in qs], tiles)
+    sources = [[q.source] for q in qs]
+    batch = bfn.Batch(sources, [[q.target] for q in qs], tiles)
+    runs = {
+        "jacobi_persistent": run(device, dg, batch, bfn.JACOBI_PULL, bfn.PERSISTENT, 0),
+        "frontier_tuned": run(device, dg, batch, bfn.FRONTIER_PUSH, bfn.CHUNKED, 0, block=256,
+                              items_per_claim=2, use_graph=1, graph_rounds=36, boundary_mode=1),
+        "dense_persistent": run(device, dg, batch, bfn.DENSE_CHAOTIC_PUSH, bfn.PERSISTENT, 0,
+                                boundary_mode=1),
+    }
+    for name, ws in runs.items():
+        assert ws.status()[0].converged == 1, name
+        failures = certify_batch(lay, ws, sources, tiles)
+        assert not failures, (name, failures[:5])
+    sample = stratified([len(t) for t in tiles]) + [i for i, q in enumerate(qs) if q.unbounded]
+    for i in sorted(set(sample)):
+        ref = oracle_region(view, lay, qs[i].source, tiles[i])
+        for name, ws in runs.items():
+            assert np.array_equal(bits(ws.region_distances(i)), bits(ref)), (name, i)
+    jac = runs["jacobi_persistent"].paths()
+    for name in ("frontier_tuned", "dense_persistent"):
+        assert [p["distance"] for p in runs[name].paths()] == [p["distance"] for p in jac], name
+    for ws in runs.values():
+        ws.close()
+    dg.close()
+
+
+def test_refused_submit_leaves_no_batch(device):
+    """A submit refused after planning (here: a tile neighbourhood larger than
+    the LDS of one workgroup) leaves the workspace without a batch -- status,
+    results and rerun report BFN_E_STATE instead of mixing the previous
+    batch's labels with the new batch's description -- and the workspace
+    takes the next valid batch as usual."""
+    hg, dg = synthetic(device, 8, 8, 64, 4)
+    view = hg.view()
+    qs = hg.synthetic_queries(3, seed=3, side_min=4, side_max=8)
+    batch = bfn.Batch([[q.source] for q in qs], [[q.target] for q in qs],
+                      [hg.query_tiles(q) for q in qs])
     ws = run(device, dg, batch, bfn.JACOBI_PULL, bfn.PERSISTENT, 0)
-    st, per, _ = ws.status()
-    assert st.converged == 1
-    # The tuned configuration: frontier push, direct boundary atomics,
-    # hipGraph replay of the rounds.
-    tuned = run(device, dg, batch, bfn.FRONTIER_PUSH, bfn.CHUNKED, 0, block=256,
-                items_per_claim=2, use_graph=1, graph_rounds=36, boundary_mode=1)
-    assert tuned.status()[0].converged == 1
-    sample = list(range(0, 256, 16)) + [i for i, q in enumerate(qs) if q.unbounded][:2]
-    for i in sample:
-        d = ws.distances(i)
-        assert certify_sssp(lay, [qs[i].source], tiles[i], d) == "", i
-        assert np.array_equal(bits(tuned.distances(i)), bits(d)), i
-    assert [p["distance"] for p in tuned.paths()] == [p["distance"] for p in ws.paths()]
-    view = dict(vertex_count=lay["vertex_count"], tile_count=lay["tile_count"],
-                row_offsets=lay["csr_row_offsets"].astype(np.uint64),
-                destinations=lay["csr_destinations"], weights=lay["csr_weights"],
-                owner_tiles=lay["owner_tiles"])
-    small = sorted(range(256), key=lambda i: len(tiles[i]))[:4]
-    for i in small:
-        ref, _ = oracle_query(view, [qs[i].source], tiles[i])
+    assert ws.status()[0].converged == 1
+    big_hg, big = synthetic(device, 4, 4, 4096, 1, host_graph=False)
+    bq = big_hg.synthetic_queries(1, seed=1, side_min=4, side_max=4)
+    big_batch = bfn.Batch([[bq[0].source]], [[bq[0].target]], [big_hg.query_tiles(bq[0])])
+    with pytest.raises(bfn.BfnError) as refused:
+        ws.submit(big, big_batch, bfn.run_options(engine=bfn.JACOBI_PULL), bfn.tuning(tile_sweeps=0))
+    assert refused.value.code == 4
+    for call in (ws.status, ws.rerun, lambda: ws.distances(0), lambda: ws.region_distances(0)):
+        with pytest.raises(bfn.BfnError) as e:
+            call()
+        assert e.value.code == 5
+    ws.submit(dg, batch, bfn.run_options(engine=bfn.FRONTIER_PUSH), bfn.tuning(tile_sweeps=0))
+    for i, q in enumerate(qs):
+        ref, _ = oracle_query(view, [q.source], batch.tiles[i])
         assert np.array_equal(bits(ws.distances(i)), bits(ref)), i
+    ws.close()
+    big.close()
+    dg.close()

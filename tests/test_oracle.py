@@ -237,3 +237,55 @@ def test_oracle_matches_reference_build_random():
             d, relax = oracle.graph_sssp(g, srcs, sel)
             rd, rrelax, _ = ref.sssp(srcs, sel)
             assert np.array_equal(bits(d), bits(rd)) and relax == rrelax
+
+
+def _cert_layout(g):
+    return dict(tile_count=g.tile_count, tile_vertex_offsets=g.tile_vertex_offsets.astype(np.uint32),
+                csc_column_offsets=g.column_offsets.astype(np.uint32), csc_sources=g.sources,
+                csc_weights=g.in_weights, owner_tiles=g.owner_tiles)
+
+
+def test_region_certificate_accepts_dijkstra_and_rejects_planted_mistakes():
+    """The size-independent certificate the GPU tests use at the big sizes:
+    it accepts the oracle's own Dijkstra labels on the full graph and on a
+    tile subset, and rejects each planted mistake: a label one ulp low, one
+    ulp high, a nonzero source, a spurious finite label on an unreachable
+    cycle of zero or absorbed weights."""
+    import bfnew_amd as bfn
+    hg = bfn.HostGraph.synthetic(8, 8, 32, 17, host_graph=True)
+    lay = hg.layout_arrays()
+    view = hg.view()
+    T = lay["tile_count"]
+    for tiles in (list(range(T)), [9, 10, 11, 17, 18, 19, 25, 26, 27]):
+        src = int(lay["tile_vertex_offsets"][tiles[len(tiles) // 2]]) + 3
+        sel = tile_mask(T, tiles)
+        d, _ = oracle.dijkstra(view["vertex_count"], view["row_offsets"], view["destinations"],
+                               view["weights"], view["owner_tiles"], sel, [src])
+        comp = d[oracle.region_vertices(lay["tile_vertex_offsets"], tiles)]
+        assert oracle.certify_region(lay, tiles, comp, [src])[0] == ""
+        finite = np.flatnonzero(np.isfinite(comp) & (comp > 0))
+        k = int(finite[len(finite) // 2])
+        for delta in (-1, 1):
+            bad = comp.copy()
+            bad.view(np.uint32)[k] = np.uint32(int(bad.view(np.uint32)[k]) + delta)
+            assert oracle.certify_region(lay, tiles, bad, [src])[0] != "", delta
+        bad = comp.copy()
+        ks = int(np.flatnonzero(comp == 0)[0])
+        bad[ks] = np.float32(1.0)
+        assert "source" in oracle.certify_region(lay, tiles, bad, [src])[0]
+    # An unreachable two-cycle with zero weights, then with weights absorbed
+    # by the label (fl(2^25 + 1) == 2^25): equal finite labels on the cycle
+    # must not certify.
+    v = plain_vertices(4)
+    for w in (0.0, 1.0):
+        e = np.zeros(3, oracle.EDGE_DTYPE)
+        for i, (s, d_, ww, r) in enumerate([(0, 1, 1.0, 1), (2, 3, w, 2), (3, 2, w, 3)]):
+            e[i] = (s, d_, ww, 1, 1, 0, r)
+        g = oracle.build_graph(v, e, None)
+        layout = _cert_layout(g)
+        src = [int(g.old_to_new[0])]
+        d, _ = oracle.graph_sssp(g, src)
+        assert oracle.certify_region(layout, [0], d, src)[0] == ""
+        bad = d.copy()
+        bad[int(g.old_to_new[2])] = bad[int(g.old_to_new[3])] = np.float32(2.0 ** 25)
+        assert "tight" in oracle.certify_region(layout, [0], bad, src)[0], w
