@@ -952,6 +952,10 @@ int32_t bfn_batch_estimate(bfn_graph graph, const bfn_batch_desc* batch,
 int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc* d,
                          const bfn_run_options* o, const bfn_tuning* tuning) {
   if (!ws || !graph) return fail(BFN_E_INVALID_ARGUMENT, "null handle");
+  // From here on the workspace holds no batch until this one is bound: a
+  // refused submit never leaves the previous batch's results readable under
+  // the new batch's description.
+  ws->submitted = ws->completed = ws->have_paths = ws->paths_counted = false;
   if (ws->device != graph->device) return fail(BFN_E_INVALID_ARGUMENT, "graph is on another device");
   if (int32_t rc = validate_options(o)) return rc;
   bfn_tuning t;
@@ -1002,7 +1006,6 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
   std::uint32_t blocks = static_cast<std::uint32_t>(per_cu) * cus;
   if (o->grid_policy == 1) blocks = std::min(blocks, o->blocks_per_wgp * cus);
 
-  ws->submitted = ws->completed = ws->have_paths = ws->paths_counted = false;
   BFN_HIP(ws->staging.reserve(meta_bytes));
   char* h = ws->staging.as<char>();
   std::memcpy(h + seg[0], d->tile_offsets, sizes[0]);

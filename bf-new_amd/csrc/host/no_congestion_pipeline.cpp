@@ -329,23 +329,6 @@ NoCongestionResultAccounting measure_no_congestion_result_transfer(std::span<con
   return a;
 }
 
-CompactQueryResult make_failed_compact_query_result(const RouteQuery& query, ExpansionQueryDisposition disposition) {
-  if (disposition == ExpansionQueryDisposition::reached)
-    throw std::invalid_argument{"a reached query has a payload, not a failure result"};
-  CompactQueryResult r;
-  r.query_id = query.query_id;
-  r.expansion_generation = query.expansion_generation;
-  r.disposition = disposition;
-  r.target_terminal_to_target = query.target_terminal_to_target;
-  for (VertexId t : query.targets) {
-    CompactTargetPath p;
-    p.summary.target = t;
-    p.summary.reconstruction = CompactPathStatus::query_terminal_failure;
-    r.targets.push_back(std::move(p));
-  }
-  return r;
-}
-
 CompactPathQualitySample sample_compact_path_quality(const WeightedGraph& graph,
                                                      std::span<const ExpansionQueryOutcome> outcomes,
                                                      std::span<const CompactQueryResult> results,

@@ -33,6 +33,42 @@ struct CompactQueryResult {
 // Every target reached with a complete path (and at least one target).
 [[nodiscard]] bool compact_path_payload_complete(const CompactPathPayload& payload) noexcept;
 
+// Host reconstruction of one query's compact paths from a graph-sized label
+// image (reference: src/compact_paths.cpp:540-640): per canonical target,
+// +inf -> unreachable; otherwise the first tight path in stable-EdgeId order
+// whose vertices all lie in the query's selected tiles, found by a
+// backtracking search that never revisits a vertex of the current path ->
+// complete, or no_tight_path when there is none.  Throws
+// std::invalid_argument for an invalid graph, query or label image (size,
+// NaN, negative, a source not at 0), std::logic_error if the payload it
+// built fails validate_compact_path_payload.
+[[nodiscard]] CompactPathPayload reconstruct_compact_path_payload(
+    const WeightedGraph& graph, const RouteQuery& query, std::span<const float> final_distances);
+
+// The payload above with its disposition, checked against the disposition
+// rules of validate_compact_query_result (std::logic_error otherwise).
+[[nodiscard]] CompactQueryResult reconstruct_compact_query_paths(
+    const WeightedGraph& graph, const RouteQuery& query, std::span<const float> final_distances,
+    ExpansionQueryDisposition disposition);
+
+// A label-free result for a query that ended in an expansion or engine
+// failure: every target query_terminal_failure.  Throws
+// std::invalid_argument for reached, unreachable_in_full_region or an
+// unknown disposition.
+[[nodiscard]] CompactQueryResult make_failed_compact_query_result(
+    const RouteQuery& query, ExpansionQueryDisposition disposition);
+
+// One result per expansion outcome, in order (outcomes sorted by query id):
+// a retained compact payload is moved out and checked (and compared with the
+// outcome's label image when it kept one); otherwise a label image is
+// reconstructed (reached / unreachable_in_full_region) or a label-free
+// failure is published.  Each outcome's payload and label image are released.
+// Throws std::invalid_argument for an invalid graph or non-canonical
+// outcomes, std::logic_error for a payload that fails its checks or a
+// reached outcome without paths or labels.
+[[nodiscard]] std::vector<CompactQueryResult> extract_host_compact_paths(
+    const WeightedGraph& graph, std::span<ExpansionQueryOutcome> outcomes);
+
 // Bytes the payloads occupy on the device-to-host wire: 28-byte summaries,
 // 4-byte vertices, labels and edge ids (edge ids are 32-bit on the device).
 // Throws std::overflow_error when a total does not fit 64 bits.

@@ -115,12 +115,6 @@ struct NoCongestionResultAccounting {
     std::span<const CompactQueryResult> results, std::uint64_t modeled_batch_status_count,
     const BatchedExpansionMetrics* device_metrics = nullptr);
 
-// A result for a query that ended without a payload (expansion failures,
-// engine failures): every target reported as query_terminal_failure.
-// std::invalid_argument for a reached disposition.
-[[nodiscard]] CompactQueryResult make_failed_compact_query_result(const RouteQuery& query,
-                                                                  ExpansionQueryDisposition disposition);
-
 enum class PathQualitySamplingMethod : std::uint8_t { splitmix64_query_id };
 
 struct CompactPathQualityObservation {

@@ -85,9 +85,67 @@ class TileDirectory {
   std::vector<EdgeOffset> halo_vertex_offsets_;
   std::vector<VertexId> halo_vertices_;
   friend TileDirectory build_tile_directory(const WeightedGraph& graph);
+  friend TileDirectory tile_directory_with_swapped_entries(const TileDirectory& directory,
+                                                           int which, std::size_t a, std::size_t b);
 };
 
 [[nodiscard]] TileDirectory build_tile_directory(const WeightedGraph& graph);
+
+// Error codes of validate_tile_directory, in the reference's order
+// (include/bfnew/spatial.hpp:126-151).
+enum class TileDirectoryValidationErrorCode : std::uint8_t {
+  none,
+  graph_validation_failed,
+  graph_is_not_spatially_ordered,
+  tile_count_mismatch,
+  spill_tile_mismatch,
+  invalid_neighbor_offsets,
+  invalid_internal_edge_offsets,
+  invalid_outgoing_cross_edge_offsets,
+  invalid_incoming_cross_edge_offsets,
+  invalid_halo_offsets,
+  neighbor_out_of_range,
+  neighbor_order_violation,
+  neighbor_asymmetry,
+  unexpected_neighbor,
+  edge_id_out_of_range,
+  edge_classification_mismatch,
+  duplicate_source_edge_classification,
+  missing_source_edge_classification,
+  duplicate_incoming_cross_edge,
+  missing_incoming_cross_edge,
+  edge_metadata_order_violation,
+  halo_vertex_out_of_range,
+  halo_owner_mismatch,
+  halo_order_violation,
+  halo_set_mismatch,
+};
+
+struct TileDirectoryValidationResult {
+  static constexpr EdgeOffset no_position = ~EdgeOffset{0};
+  TileDirectoryValidationErrorCode code{TileDirectoryValidationErrorCode::none};
+  EdgeOffset position{no_position};
+  [[nodiscard]] constexpr bool ok() const noexcept {
+    return code == TileDirectoryValidationErrorCode::none;
+  }
+};
+
+// Deep check of a directory against its graph (reference behaviour:
+// src/tiled_graph.cpp:293-484): group offsets; neighbour groups sorted,
+// symmetric, and exactly the Chebyshev-1 located neighbours plus the tiles
+// that share an edge with the spill tile; every edge classified once by its
+// source tile (internal or outgoing, in CSR order) and every cross edge once
+// by its destination tile (in CSC order); halos exactly the remote endpoints
+// of each tile's cross edges, ordered by (owner tile, vertex).
+[[nodiscard]] TileDirectoryValidationResult validate_tile_directory(const WeightedGraph& graph,
+                                                                    const TileDirectory& directory);
+
+// Test hook: a copy of `directory` whose group `which` (0 neighbours,
+// 1 internal, 2 outgoing, 3 incoming, 4 halo) has entries `a` and `b`
+// swapped, so validation tests can plant a mistake without private access.
+[[nodiscard]] TileDirectory tile_directory_with_swapped_entries(const TileDirectory& directory,
+                                                                int which, std::size_t a,
+                                                                std::size_t b);
 
 class SpatialPartitioner {
  public:

@@ -335,8 +335,10 @@ def test_refused_submit_leaves_no_batch(device):
                       [hg.query_tiles(q) for q in qs])
     ws = run(device, dg, batch, bfn.JACOBI_PULL, bfn.PERSISTENT, 0)
     assert ws.status()[0].converged == 1
-    big_hg, big = synthetic(device, 4, 4, 4096, 1, host_graph=False)
-    bq = big_hg.synthetic_queries(1, seed=1, side_min=4, side_max=4)
+    # 8x8 tiles of 4096 vertices: an interior tile reads 13 in-neighbour slabs
+    # (itself and +-1, +-2, +-6 in x and y), 13 * 16 KiB > the 160 KiB of LDS.
+    big_hg, big = synthetic(device, 8, 8, 4096, 1, host_graph=False)
+    bq = big_hg.synthetic_queries(1, seed=1, side_min=8, side_max=8)
     big_batch = bfn.Batch([[bq[0].source]], [[bq[0].target]], [big_hg.query_tiles(bq[0])])
     with pytest.raises(bfn.BfnError) as refused:
         ws.submit(big, big_batch, bfn.run_options(engine=bfn.JACOBI_PULL), bfn.tuning(tile_sweeps=0))
