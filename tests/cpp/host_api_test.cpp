@@ -8,6 +8,10 @@
 //                                  reference tests' literal fixtures
 //                                  (sssp_test.cpp:108-227,
 //                                  compact_paths_test.cpp:114-205)
+//   host_api_test workspace        batch_workspace byte model, reservations
+//                                  and decisions on the reference's workspace
+//                                  fixture (tests/batch_workspace_test.cpp:
+//                                  77-960 literal totals)
 //   host_api_test paths-file IN OUT
 //                                  reconstructs paths for the graph, labels
 //                                  and targets in IN (written by
@@ -17,6 +21,7 @@
 //
 // Labels of the literal fixtures come from a Bellman-Ford written here (test
 // code: the library has no CPU shortest-path engine).
+#include "bfnew/batch_workspace.hpp"
 #include "bfnew/batched_expansion.hpp"
 #include "bfnew/compact_paths.hpp"
 #include "bfnew/graph.hpp"
@@ -452,6 +457,296 @@ void paths() {
 }
 
 // ---------------------------------------------------------------------------
+// Workspace byte model (batch_workspace_test.cpp:77-960).
+// ---------------------------------------------------------------------------
+struct WorkspaceFixture {
+  PartitionedGraph partitioned;
+  TileRunLayout64 tile_runs;
+  std::vector<RouteQuery> dense, sparse;
+};
+
+WorkspaceFixture workspace_fixture() {
+  std::vector<VertexMetadata> v{
+      VertexMetadata::located(0, 0, kClass1),  VertexMetadata::located(1, 0, kClass1),
+      VertexMetadata::located(10, 0, kClass1), VertexMetadata::located(30, 0, kClass1),
+      VertexMetadata::unlocated(kClass1),      VertexMetadata::located(11, 0, kClass1),
+      VertexMetadata::located(2, 0, kClass1)};
+  std::vector<EdgeInputRecord> e{edge(0, 1, 1.0F, 0), edge(0, 2, 2.0F, 1), edge(0, 3, 3.0F, 2),
+                                 edge(0, 3, 4.0F, 3), edge(0, 4, 5.0F, 4), edge(4, 0, 6.0F, 5),
+                                 edge(2, 0, 7.0F, 6), edge(1, 2, 8.0F, 7)};
+  const UniformGridPartitioner partitioner{SpatialOrderConfig{0, 0, 10U, 10U}};
+  WorkspaceFixture f{partitioner.partition(InputGraph{std::move(v), std::move(e)}), {}, {}, {}};
+  const WeightedGraph& g = f.partitioned.graph;
+  auto vid = [&](std::uint32_t old) { return g.old_to_new()[old]; };
+  auto tile = [&](std::uint32_t old) { return g.owner_tiles()[vid(old).value()]; };
+  auto query = [&](std::uint32_t id, std::vector<std::uint32_t> s, std::vector<std::uint32_t> t,
+                   std::uint32_t generation, std::vector<TileId> tiles) {
+    std::vector<VertexId> sv, tv;
+    for (auto x : s) sv.push_back(vid(x));
+    for (auto x : t) tv.push_back(vid(x));
+    RouteQuery q = make_route_query(QueryId{id}, g, sv, tv, 0U, generation);
+    std::sort(tiles.begin(), tiles.end());
+    q.selected_tiles = std::move(tiles);
+    if (!validate_route_query(g, q).ok()) throw std::runtime_error{"invalid workspace fixture query"};
+    return q;
+  };
+  const TileId t0 = tile(0), t1 = tile(2), t2 = tile(3), spill = tile(4);
+  f.dense = {query(10, {0, 2}, {5}, 0, {t0, t1}), query(20, {0}, {4}, 1, {t0, spill}),
+             query(30, {4}, {4}, 2, {spill}), query(40, {2}, {3}, 3, {t1, t2})};
+  f.sparse = {query(50, {2}, {5}, 0, {t1}), query(60, {4}, {4}, 0, {spill})};
+  f.tile_runs = build_tile_run_layout(g);
+  return f;
+}
+
+struct PreparedBatch {
+  std::vector<BatchQueryFeatures> features;
+  BatchPlan plan;
+  BatchDeviceDescription retained, descriptors;
+};
+
+// Cold preparation, then the same preparation again (the warmed reusable
+// path the reported write counters model).
+PreparedBatch prepare_warm(const WorkspaceFixture& f, const SelectedRegionIndex& regions,
+                           std::span<const RouteQuery> queries, std::uint32_t width, bool zero_overlap) {
+  const WeightedGraph& g = f.partitioned.graph;
+  PreparedBatch p;
+  p.features = make_batch_query_features(g, regions, queries);
+  BatchPlannerPolicy policy;
+  policy.lane_width = width;
+  policy.maximum_union_inflation_numerator = 3U;
+  if (zero_overlap) {
+    policy.minimum_jaccard_numerator = 0U;
+    policy.minimum_jaccard_denominator = 1U;
+  }
+  p.plan = make_overlapping_batch_plan(regions, p.features, policy);
+  if (p.plan.batches.size() != 1U) throw std::runtime_error{"fixture split into several batches"};
+  const BatchPlanEntry& b = p.plan.batches.front();
+  for (int pass = 0; pass < 2; ++pass) {
+    prepare_batch_device_description(g, f.tile_runs, queries, p.features, b,
+                                     BatchRunRepresentation::retained_per_run_masks, p.retained);
+    prepare_batch_device_description(g, f.tile_runs, queries, p.features, b,
+                                     BatchRunRepresentation::compact_nonzero_descriptors, p.descriptors);
+    prepare_compact_vertex_mapping(g, b, p.retained);
+    prepare_compact_vertex_mapping(g, b, p.descriptors);
+  }
+  if (!validate_compact_vertex_mapping(g, b, p.retained).ok() ||
+      !validate_compact_vertex_mapping(g, b, p.descriptors).ok())
+    throw std::runtime_error{"fixture compact mapping is invalid"};
+  return p;
+}
+
+const BatchWorkspaceEstimate& row(const std::vector<BatchWorkspaceEstimate>& rows, BatchVertexStorageStrategy s,
+                                  BatchRunRepresentation r) {
+  for (const auto& e : rows)
+    if (e.vertex_storage == s && e.run_representation == r) return e;
+  throw std::runtime_error{"missing strategy row"};
+}
+
+void workspace() {
+  using S = BatchVertexStorageStrategy;
+  using R = BatchRunRepresentation;
+  const WorkspaceFixture f = workspace_fixture();
+  const WeightedGraph& g = f.partitioned.graph;
+  const SelectedRegionIndex regions{g, f.tile_runs};
+
+  // Dense matrix at widths 8/16/32 and one or two slots
+  // (batch_workspace_test.cpp:275-378).
+  const BatchWorkspaceBudget budget{1'000'000U, 1'000U, 500U};
+  for (const std::uint32_t width : {8U, 16U, 32U}) {
+    const PreparedBatch p = prepare_warm(f, regions, f.dense, width, false);
+    const BatchPlanEntry& b = p.plan.batches.front();
+    CHECK(b.union_vertex_count == 7U && b.selected_lane_vertex_count == 13U);
+    const std::uint64_t metadata = 32U * static_cast<std::uint64_t>(width) + 280U;
+    for (const std::uint32_t slots : {1U, 2U}) {
+      const auto rows = compare_batch_workspace_strategies(g, f.tile_runs, b, p.retained, p.descriptors, slots, budget);
+      CHECK(rows.size() == 4U);
+      const std::uint64_t dist = 7U * static_cast<std::uint64_t>(width) * slots * 4U;
+      const std::uint64_t reset = 13U * slots * 4U;
+      for (const S storage : {S::full_graph_vertex_major, S::compact_union_tiles}) {
+        const bool compact = storage == S::compact_union_tiles;
+        const auto& ret = row(rows, storage, R::retained_per_run_masks);
+        const auto& des = row(rows, storage, R::compact_nonzero_descriptors);
+        CHECK(ret.distance_bytes == dist && des.distance_bytes == dist);
+        CHECK(ret.distance_reset_bytes == reset && des.distance_reset_bytes == reset);
+        CHECK(ret.allocated_lane_vertex_count == 7U * width && ret.wasted_lane_vertex_count == 7U * width - 13U);
+        CHECK(ret.batch_metadata_bytes == metadata && des.batch_metadata_bytes == metadata);
+        CHECK(ret.run_storage_bytes == 28U && ret.descriptor_offset_bytes == 0U &&
+              ret.tile_mapping_bytes == (compact ? 16U : 0U));
+        CHECK(des.run_storage_bytes == 48U && des.descriptor_offset_bytes == 32U &&
+              des.tile_mapping_bytes == (compact ? 16U : 0U));
+        CHECK(ret.active_csr_runs == 6U && ret.active_csc_runs == 5U && ret.zero_csr_runs == 1U &&
+              ret.zero_csc_runs == 1U && des.active_csr_runs == 6U && des.active_csc_runs == 5U);
+        CHECK(ret.run_preparation_write_bytes == 48U && des.run_preparation_write_bytes == 80U);
+        CHECK(ret.tile_mapping_write_bytes == (compact ? 32U : 0U) &&
+              des.tile_mapping_write_bytes == (compact ? 32U : 0U));
+        CHECK(ret.total_workspace_bytes == dist + (compact ? 16U : 0U) + 28U + metadata);
+        CHECK(des.total_workspace_bytes == dist + (compact ? 16U : 0U) + 48U + 32U + metadata);
+        CHECK(ret.maximum_concurrent_workspaces == (1'000'000U - 1'500U) / ret.total_workspace_bytes);
+      }
+    }
+  }
+
+  // Sparse, noncontiguous compact union (batch_workspace_test.cpp:380-488).
+  {
+    const PreparedBatch p = prepare_warm(f, regions, f.sparse, 8U, true);
+    const BatchPlanEntry& b = p.plan.batches.front();
+    CHECK(b.union_tiles == (std::vector<TileId>{TileId{1U}, TileId{3U}}) && b.union_vertex_count == 3U &&
+          b.selected_lane_vertex_count == 3U);
+    CHECK(p.retained.touched_compact_tiles == (std::vector<std::uint32_t>{1U, 3U}));
+    CHECK(p.retained.compact_vertex_biases_by_tile == (std::vector<std::uint32_t>{0U, 3U, 0U, 4U}));
+    BatchDeviceDescription stale = p.retained;
+    stale.compact_vertex_biases_by_tile[0] = 9U;
+    CHECK(validate_compact_vertex_mapping(g, b, stale).code ==
+          BatchLayoutValidationErrorCode::invalid_compact_vertex_mapping);
+    CHECK(p.retained.run_report.active_csr_runs == 0U && p.retained.run_report.active_csc_runs == 0U &&
+          p.descriptors.csr_run_descriptors.empty() && p.descriptors.csc_run_descriptors.empty());
+    const BatchWorkspaceBudget small{100'000U, 0U, 0U};
+    const auto rows = compare_batch_workspace_strategies(g, f.tile_runs, b, p.retained, p.descriptors, 2U, small);
+    BatchDeviceDescription cold;
+    prepare_batch_device_description(g, f.tile_runs, f.sparse, p.features, b, R::retained_per_run_masks, cold);
+    prepare_compact_vertex_mapping(g, b, cold);
+    const auto c = estimate_batch_workspace(g, f.tile_runs, b, cold, S::compact_union_tiles,
+                                            R::retained_per_run_masks, 2U, small);
+    CHECK(c.tile_mapping_write_bytes == 24U && c.run_preparation_write_bytes == 28U &&
+          c.total_preparation_write_bytes == 76U);
+    const auto& fr = row(rows, S::full_graph_vertex_major, R::retained_per_run_masks);
+    const auto& cr = row(rows, S::compact_union_tiles, R::retained_per_run_masks);
+    const auto& fd = row(rows, S::full_graph_vertex_major, R::compact_nonzero_descriptors);
+    const auto& cd = row(rows, S::compact_union_tiles, R::compact_nonzero_descriptors);
+    CHECK(fr.distance_bytes == 448U && cr.distance_bytes == 192U && fr.wasted_lane_vertex_count == 53U &&
+          cr.wasted_lane_vertex_count == 21U);
+    CHECK(fr.distance_reset_bytes == 24U && cr.distance_reset_bytes == 24U);
+    CHECK(fr.total_workspace_bytes == 960U && cr.total_workspace_bytes == 720U &&
+          fd.total_workspace_bytes == 948U && cd.total_workspace_bytes == 708U);
+  }
+
+  // Budget boundaries, rejections, overflow and reuse (:490-670).
+  const PreparedBatch p = prepare_warm(f, regions, f.dense, 8U, false);
+  const BatchPlanEntry& b = p.plan.batches.front();
+  const BatchWorkspaceBudget generous{1'000'000U, 0U, 0U};
+  auto estimate = [&](const BatchPlanEntry& batch, const BatchDeviceDescription& d, S s, R r, std::uint32_t slots,
+                      const BatchWorkspaceBudget& budget_) {
+    return estimate_batch_workspace(g, f.tile_runs, batch, d, s, r, slots, budget_);
+  };
+  const auto base = estimate(b, p.retained, S::full_graph_vertex_major, R::retained_per_run_masks, 2U, generous);
+  const std::uint64_t bytes = base.total_workspace_bytes;
+  for (const auto& [capacity, expected] : {std::pair{bytes - 1U, 0U}, std::pair{bytes, 1U},
+                                           std::pair{3U * bytes - 1U, 2U}, std::pair{3U * bytes, 3U}})
+    CHECK(estimate(b, p.retained, S::full_graph_vertex_major, R::retained_per_run_masks, 2U,
+                   BatchWorkspaceBudget{capacity, 0U, 0U})
+              .maximum_concurrent_workspaces == expected);
+  CHECK(throws<std::invalid_argument>(
+      [&] { (void)estimate(b, p.retained, S::full_graph_vertex_major, R::retained_per_run_masks, 0U, generous); }));
+  CHECK(throws<std::invalid_argument>(
+      [&] { (void)estimate(b, p.retained, static_cast<S>(255U), R::retained_per_run_masks, 2U, generous); }));
+  CHECK(throws<std::invalid_argument>(
+      [&] { (void)estimate(b, p.retained, S::full_graph_vertex_major, static_cast<R>(255U), 2U, generous); }));
+  BatchDeviceDescription unmapped = p.retained;
+  unmapped.compact_vertex_mapping_valid = false;
+  CHECK(throws<std::invalid_argument>(
+      [&] { (void)estimate(b, unmapped, S::compact_union_tiles, R::retained_per_run_masks, 2U, generous); }));
+  CHECK(throws<std::overflow_error>([&] {
+    (void)estimate(b, p.retained, S::full_graph_vertex_major, R::retained_per_run_masks, 2U,
+                   BatchWorkspaceBudget{~0ULL, ~0ULL, 1U});
+  }));
+  BatchPlanEntry overflow = b;
+  overflow.union_vertex_count = ~0ULL;
+  overflow.selected_lane_vertex_count = 0U;
+  CHECK(throws<std::overflow_error>(
+      [&] { (void)estimate(overflow, p.retained, S::compact_union_tiles, R::retained_per_run_masks, 2U, generous); }));
+  BatchPlanEntry impossible = b;
+  impossible.selected_lane_vertex_count = static_cast<std::uint64_t>(g.vertex_count()) * b.lane_width + 1U;
+  CHECK(throws<std::invalid_argument>([&] {
+    (void)estimate(impossible, p.retained, S::full_graph_vertex_major, R::retained_per_run_masks, 2U, generous);
+  }));
+  ReusableBatchWorkspaceReservation reservation;
+  const auto first = reservation.reserve(base);
+  const auto repeated = reservation.reserve(base);
+  CHECK(first.capacity_grew && !repeated.capacity_grew && first.generation == 1U && repeated.generation == 2U &&
+        reservation.growth_events() == 1U);
+  const PreparedBatch wide = prepare_warm(f, regions, f.dense, 16U, false);
+  const auto larger = estimate(wide.plan.batches.front(), wide.retained, S::full_graph_vertex_major,
+                               R::retained_per_run_masks, 2U, generous);
+  const auto grown = reservation.reserve(larger);
+  CHECK(grown.capacity_grew && grown.generation == 3U && reservation.growth_events() == 2U &&
+        reservation.capacity().total_bytes() >= larger.total_workspace_bytes);
+  BatchWorkspaceEstimate inconsistent = base;
+  ++inconsistent.total_workspace_bytes;
+  CHECK(throws<std::invalid_argument>([&] { (void)reservation.reserve(inconsistent); }));
+
+  // Device-materialised Jacobi rows (:672-748).
+  {
+    BatchDeviceDescription dm;
+    prepare_batch_device_description(g, f.tile_runs, f.dense, p.features, b, R::device_materialized_run_masks, dm);
+    const auto e = estimate(b, dm, S::full_graph_vertex_major, R::device_materialized_run_masks, 2U, generous);
+    CHECK(e.run_storage_bytes == f.tile_runs.csc_run_source_tiles.size() * 4U && e.descriptor_offset_bytes == 0U &&
+          e.active_csr_runs == 0U && e.zero_csr_runs == 0U &&
+          e.active_csc_runs == p.retained.run_report.active_csc_runs &&
+          e.zero_csc_runs + e.active_csc_runs == f.tile_runs.csc_run_source_tiles.size());
+    CHECK(e.run_preparation_write_bytes == p.retained.run_report.csc_runs_visited * 4U && e.run_build_nanoseconds == 0U);
+    CHECK(throws<std::invalid_argument>([&] {
+      (void)estimate_batch_workspace(g, f.tile_runs, b, dm, S::full_graph_vertex_major,
+                                     R::device_materialized_run_masks, 2U, generous, {0U, 1U});
+    }));
+    CHECK(throws<std::invalid_argument>(
+        [&] { (void)estimate(b, dm, S::compact_union_tiles, R::device_materialized_run_masks, 2U, generous); }));
+    CHECK(throws<std::invalid_argument>(
+        [&] { (void)estimate(b, dm, S::full_graph_vertex_major, R::device_materialized_run_masks, 1U, generous); }));
+  }
+
+  // Warmed preparation counters and the recorded decision (:750-960).
+  CHECK(p.retained.compact_mapping_report.entries_cleared == 4U &&
+        p.retained.compact_mapping_report.entries_written == 4U &&
+        p.retained.run_report.csr_retained_entries_cleared == 6U &&
+        p.retained.run_report.csr_retained_entries_written == 6U);
+  CHECK(p.retained.run_report.csr_lane_edge_pairs == 7U && p.retained.run_report.csc_lane_edge_pairs == 7U &&
+        p.descriptors.run_report.csr_lane_edge_pairs == 7U && p.descriptors.run_report.csc_lane_edge_pairs == 7U);
+  const BatchWorkspaceBudget decision_budget{65'536U, 4'096U, 4'096U};
+  const BatchWorkspaceBuildMeasurements measured{11U, 22U, 33U};
+  const auto rows = compare_batch_workspace_strategies(g, f.tile_runs, b, p.retained, p.descriptors, 2U,
+                                                       decision_budget, measured);
+  for (const auto& e : rows) {
+    const bool compact = e.vertex_storage == S::compact_union_tiles;
+    const bool retained = e.run_representation == R::retained_per_run_masks;
+    CHECK(e.mapping_build_nanoseconds == (compact ? 11U : 0U) && e.run_build_nanoseconds == (retained ? 22U : 33U));
+  }
+  CHECK(row(rows, S::full_graph_vertex_major, R::retained_per_run_masks).total_preparation_write_bytes == 152U);
+  CHECK(row(rows, S::full_graph_vertex_major, R::compact_nonzero_descriptors).total_preparation_write_bytes == 184U);
+  CHECK(row(rows, S::compact_union_tiles, R::retained_per_run_masks).total_preparation_write_bytes == 184U);
+  CHECK(row(rows, S::compact_union_tiles, R::compact_nonzero_descriptors).total_preparation_write_bytes == 216U);
+  BatchWorkspaceDecision decision;
+  decision.plan_fingerprint = fingerprint_batch_plan(p.plan);
+  decision.budget = decision_budget;
+  decision.graph_vertex_count = g.vertex_count();
+  decision.graph_tile_count = g.tile_coordinates().size();
+  decision.lane_width = 8U;
+  decision.distance_slot_count = 2U;
+  decision.compared_strategies = rows;
+  decision.selected_run_build_nanoseconds = 22U;
+  decision.quantitative_reason = "full/retained uses the fewest bytes and needs no mapping table";
+  CHECK(validate_batch_workspace_decision(decision));
+  const std::string text = serialize_batch_workspace_decision(decision);
+  CHECK(text.rfind("bfnew_phase13_workspace_decision_v1\nplan_fingerprint=", 0) == 0);
+  CHECK(text.find("selected_vertex_storage=full_graph_vertex_major\n") != std::string::npos);
+  CHECK(std::count(text.begin(), text.end(), '\n') == 17 + 4);  // 16 header lines, column names, 4 rows
+  // Each planted inconsistency is refused.
+  auto refused = [&](auto&& plant) {
+    BatchWorkspaceDecision d = decision;
+    plant(d);
+    return !validate_batch_workspace_decision(d) &&
+           throws<std::invalid_argument>([&] { (void)serialize_batch_workspace_decision(d); });
+  };
+  CHECK(refused([](BatchWorkspaceDecision& d) { d.quantitative_reason = "tab\there"; }));
+  CHECK(refused([](BatchWorkspaceDecision& d) { d.compared_strategies.pop_back(); }));
+  CHECK(refused([](BatchWorkspaceDecision& d) { d.compared_strategies[1] = d.compared_strategies[0]; }));
+  CHECK(refused([](BatchWorkspaceDecision& d) { ++d.compared_strategies[0].distance_bytes; }));
+  CHECK(refused([](BatchWorkspaceDecision& d) { ++d.compared_strategies[2].run_storage_bytes; }));
+  CHECK(refused([](BatchWorkspaceDecision& d) { d.selected_run_build_nanoseconds = 23U; }));
+  CHECK(refused([](BatchWorkspaceDecision& d) { d.budget.device_capacity_bytes = 8'192U; }));
+  CHECK(refused([](BatchWorkspaceDecision& d) { d.plan_fingerprint = 0U; }));
+}
+
+// ---------------------------------------------------------------------------
 // Paths for a graph written by the Python test (oracle labels).
 //   IN : u32 V, u32 E, then V x (i32 x, i32 y, u32 located, u32 class),
 //        E x (u32 src, u32 dst, f32 w, u64 record), u32 nsrc, nsrc x u32
@@ -522,10 +817,12 @@ int main(int argc, char** argv) {
       tiles();
     } else if (mode == "paths") {
       paths();
+    } else if (mode == "workspace") {
+      workspace();
     } else if (mode == "paths-file" && argc == 4) {
       paths_file(argv[2], argv[3]);
     } else {
-      std::fprintf(stderr, "usage: host_api_test tiles|paths|paths-file IN OUT\n");
+      std::fprintf(stderr, "usage: host_api_test tiles|paths|workspace|paths-file IN OUT\n");
       return 2;
     }
   } catch (const std::exception& ex) {

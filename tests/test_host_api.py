@@ -9,6 +9,13 @@
   make_failed_compact_query_result, extract_host_compact_paths) on the
   literal fixtures of the reference's sssp_test.cpp:108-227 and
   compact_paths_test.cpp:114-205 (zero cycle, EdgeId tie, bounded region).
+- workspace: the batch_workspace byte model (estimate_batch_workspace, the
+  2 x 2 strategy comparison, reservations, decisions and their record) on the
+  reference's workspace fixture, against the literal totals of its
+  tests/batch_workspace_test.cpp:275-960 (dense widths 8/16/32, the sparse
+  compact union 960/720/948/708 bytes, cold 24/28/76 preparation bytes,
+  warmed 152/184/184/216, budget boundaries, overflow and planted decision
+  mistakes).
 - random graphs with zero-weight edges (so the search backtracks): the
   library's reconstruction equals the oracle's path by path, from oracle
   labels.
@@ -41,6 +48,10 @@ def test_tile_directory_reference_literals():
 
 def test_path_reconstruction_reference_literals():
     _run("paths")
+
+
+def test_batch_workspace_reference_literals():
+    _run("workspace")
 
 
 @pytest.mark.parametrize("seed", [1, 2, 3])
