@@ -102,17 +102,28 @@ def apply_tuning(batch, qs, tune_entry):
     return t
 
 
-def timed_steps(ws, steps, warmup):
+def timed_steps(ws, steps, warmup, paths=False):
+    """Times `steps` reruns of the resident batch (with paths: each followed
+    by compact-path reconstruction of every target and the copy of the
+    summaries and path arenas to the host).  Returns the wall time, the
+    relaxation time of each step (HIP events on the workspace stream), the
+    round launches of the last step and the wall time of each paths call."""
     for _ in range(warmup):
         ws.rerun()
+        if paths:
+            ws.paths()
     bfn.hip_lib().bfn_device_synchronize(ws.device.h)
-    relax_ms = []
+    relax_ms, paths_ms = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
         ws.rerun()
         relax_ms.append(ws.timing()[0])
+        if paths:
+            t = time.perf_counter()
+            ws.paths()
+            paths_ms.append((time.perf_counter() - t) * 1e3)
     bfn.hip_lib().bfn_device_synchronize(ws.device.h)
-    return time.perf_counter() - t0, relax_ms, ws.timing()[2]
+    return time.perf_counter() - t0, relax_ms, ws.timing()[2], paths_ms
 
 
 def cpu_baseline(hg_view, qs, tiles, budget_s):
@@ -171,7 +182,11 @@ def cpu_baseline(hg_view, qs, tiles, budget_s):
 
 
 def plain_vs_batched(dev, steps):
-    """Config 2: 64 queries one per launch (plain) vs all 64 in one launch."""
+    """Config 2: 64 queries one per launch (plain) vs all 64 in one launch.
+    Plain mode gives each query its own workspace and runs it as one
+    persistent cooperative launch with the same engine knobs as the batched
+    run (tile closure, direct boundary pushes): the launch count, not the
+    engine, is what differs."""
     hg = bfn.HostGraph.synthetic(*C2["tiles"], C2["seed"])
     dg = bfn.DeviceGraph(dev, hg.layout())
     qs, batch = make_batch(hg, C2)
@@ -180,7 +195,7 @@ def plain_vs_batched(dev, steps):
               bfn.tuning(tile_sweeps=0, count_relaxations=1))
     _, per, _ = ws.status()
     relax = [p.relaxations for p in per]
-    elapsed, _, _ = timed_steps(ws, steps, 1)
+    elapsed, _, _, _ = timed_steps(ws, steps, 1)
     batched = sum(relax) * steps / elapsed
     out = {}
     for eng in (bfn.JACOBI_PULL, bfn.DENSE_CHAOTIC_PUSH, bfn.FRONTIER_PUSH):
@@ -188,8 +203,8 @@ def plain_vs_batched(dev, steps):
         for i in range(len(qs)):
             one = bfn.Batch([[qs[i].source]], [[qs[i].target]], [batch.tiles[i]])
             w = bfn.Workspace(dev)
-            w.submit(dg, one, bfn.run_options(engine=eng, control_mode=bfn.PER_ROUND),
-                     bfn.tuning(tile_sweeps=1))
+            w.submit(dg, one, bfn.run_options(engine=eng, control_mode=bfn.PERSISTENT),
+                     bfn.tuning(tile_sweeps=0, boundary_mode=1))
             wss.append(w)
         lat = []
         t0 = time.perf_counter()
@@ -199,8 +214,8 @@ def plain_vs_batched(dev, steps):
             lat.append(time.perf_counter() - t)
         elapsed_plain = time.perf_counter() - t0
         out[bfn.ENGINE_NAMES[eng]] = dict(
-            value=sum(relax) / elapsed_plain, p50_ms=float(np.percentile(lat, 50) * 1e3),
-            p99_ms=float(np.percentile(lat, 99) * 1e3))
+            value=sum(relax) / elapsed_plain, p50_ms=percentile(lat, 50) * 1e3,
+            p99_ms=percentile(lat, 99) * 1e3, control="persistent_cooperative, one launch per query")
         for w in wss:
             w.close()
     best_plain = max(v["value"] for v in out.values())
@@ -209,7 +224,7 @@ def plain_vs_batched(dev, steps):
 
 
 SERVING = dict(name="c4_serving_64x64x256", tiles=(64, 64, 256), seed=2, query_seed=44,
-               rates=(2000, 10000, 50000), max_requests=4000)
+               rates=(2000, 10000, 50000), max_requests=2000)
 
 
 def percentile(values, p):
@@ -360,8 +375,8 @@ def main():
     ap.add_argument("--cpu-budget-s", type=float, default=10.0)
     ap.add_argument("--negotiated", action="store_true",
                     help="also run the negotiated-cost rounds on the 256x256x512 graph")
-    ap.add_argument("--serving", action="store_true",
-                    help="also run the Poisson-arrival serving measurement")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the Poisson-arrival serving measurement")
     ap.add_argument("--expansion", action="store_true",
                     help="also time the expansion orchestrator over the device executor")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -416,7 +431,7 @@ def main():
                                           graph_rounds=tune["graph_rounds"],
                                           boundary_mode=tune["boundary_mode"],
                                           min_waves=tune["min_waves"]))
-    elapsed, relax_ms, launches = timed_steps(ws, args.steps, args.warmup)
+    elapsed, relax_ms, launches, paths_ms = timed_steps(ws, args.steps, args.warmup, paths=True)
     # one round kernel + one advance kernel per host-polled or replayed round
     round_launches = 1 if tune["control"] == bfn.PERSISTENT and not tune["use_graph"] else launches // 2
     value = relax_per_step * args.steps / elapsed
@@ -424,6 +439,7 @@ def main():
     kernel_ms = float(np.mean(relax_ms))
     achieved = counters.algorithmic_bytes / (kernel_ms * 1e-3) / 1e9
     log(f"{value / 1e9:.2f} G relax/s, {ms_per_step:.2f} ms/step, relax kernel {kernel_ms:.3f} ms, "
+        f"compact paths {float(np.mean(paths_ms)):.3f} ms, "
         f"{counters.tile_visits} tile visits, {counters.rounds_executed} rounds")
     traffic, traffic_source = profiled_traffic(C3["name"], tune)
     cycles = [int(c) for c in counters.phase_cycles]
@@ -458,7 +474,10 @@ def main():
                    "min_waves": tune["min_waves"],
                    "control": bfn.CONTROL_NAMES[tune["control"]] +
                    (f"+hipgraph({tune['graph_rounds']} rounds)" if tune["use_graph"] else ""),
-                   "relaxations_per_step": relax_per_step},
+                   "relaxations_per_step": relax_per_step,
+                   "step": "label initialisation + relaxation to the fixed point + per-query "
+                           "finalisation + compact paths of every target copied to the host",
+                   "compact_paths_ms_per_step": float(np.mean(paths_ms))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_source,
@@ -476,8 +495,10 @@ def main():
         pm = plain_vs_batched(dev, max(3, args.steps // 2))
         result["plain_mode"] = pm
         log("plain vs batched (C2):", json.dumps(pm))
-    if args.serving:
-        result["serving"] = serving(dev)
+    if not args.no_serving:
+        sv = serving(dev)
+        result["serving"] = sv
+        result["p99_query_latency_ms"] = {r: sv[r]["p99_ms"] for r in map(str, SERVING["rates"])}
     if args.negotiated:
         result["negotiated"] = negotiated(dev, dict(tune, control=bfn.PERSISTENT))
     if expansion_result is not None:

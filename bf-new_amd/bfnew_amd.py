@@ -215,6 +215,9 @@ HIP_SIGNATURES = {
     "bfn_server_submit": ([C.c_void_p, C.POINTER(Request)], C.c_int32),
     "bfn_server_poll": ([C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32],
                         C.c_int32),
+    "bfn_server_poll_paths": ([C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32),
+                               C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p],
+                              C.c_int32),
     "bfn_server_stats": ([C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                           C.POINTER(C.c_uint64)], C.c_int32),
     "bfn_server_destroy": ([C.c_void_p], C.c_int32),
@@ -704,6 +707,26 @@ class Server:
                      path_length=buf[i].path_length, reached=buf[i].reached,
                      submit_ns=buf[i].submit_ns, complete_ns=buf[i].complete_ns)
                 for i in range(n.value)]
+
+    def poll_paths(self, capacity=1024, timeout_ms=100, vertex_capacity=1 << 22):
+        """poll() with each response's compact path: 'vertices' (source
+        first) and 'edge_ids' (empty when unreached)."""
+        buf = (Response * capacity)()
+        verts = np.zeros(vertex_capacity, np.uint32)
+        edges = np.zeros(vertex_capacity, np.uint32)
+        offs = np.zeros(capacity + 1, np.uint64)
+        n = C.c_uint32()
+        _check(hip_lib().bfn_server_poll_paths(self.h, buf, capacity, C.byref(n), timeout_ms,
+                                               _ptr(verts), _ptr(edges), vertex_capacity,
+                                               _ptr(offs)))
+        out = []
+        for i in range(n.value):
+            a, b = int(offs[i]), int(offs[i + 1])
+            out.append(dict(request_id=buf[i].request_id, distance=buf[i].distance,
+                            path_length=buf[i].path_length, reached=buf[i].reached,
+                            submit_ns=buf[i].submit_ns, complete_ns=buf[i].complete_ns,
+                            vertices=verts[a:b].copy(), edge_ids=edges[a:max(a, b - 1)].copy()))
+        return out
 
     def stats(self):
         a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()

@@ -50,10 +50,16 @@ constexpr std::uint32_t kMaxTileVertices = 4096;  // 12-bit local ids
 constexpr std::uint32_t kMaxNeighborTiles = 255;  // 8-bit neighbour index
 constexpr std::uint32_t kPackLocalMask = 0xFFFu;
 constexpr int kPhaseCount = 5;  // visit phases timed by instrumented runs
-// LDS table of a visit: neighbour slots [0, 256), pushed-into marks
-// [256, 512), proposal slab offsets [512, 768); then the flag words.
-constexpr std::uint32_t kLdsTableWords = 1024;
+// LDS table of a visit, sized by the graph's largest neighbour-tile list K
+// (rounded up to 4): neighbour slots [0, K), pushed-into marks [K, 2K),
+// per-neighbour offsets [2K, 4K) (two words each: a pool offset, or one
+// word: a staged slab offset); then the flag words.
 constexpr std::uint32_t kLdsFlagWords = 8;
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8), so
+// a barrier that first gathers each XCD's workgroups spreads its arrivals
+// over 8 counters instead of one.
+constexpr std::uint32_t kBarrierGroups = 8;
+constexpr std::uint32_t kBarrierStride = 32;  // words: one 128-byte line per group
 
 enum Engine : std::uint32_t { kJacobiPull = 0, kDensePush = 1, kFrontierPush = 2 };
 
@@ -75,6 +81,8 @@ struct DevGraph {
   std::uint32_t max_in_edges;
   std::uint32_t max_out_edges;
   std::uint32_t max_out_stage_floats;  // largest out-neighbour proposal area (k >= 1)
+  std::uint32_t max_neighbor_tiles;    // largest in- or out-neighbour tile list (<= 255)
+  std::uint32_t pad0;
   const std::uint32_t* owner_tiles;
   const std::uint32_t* tile_vertex_offsets;
   const std::uint32_t* row_offsets;
@@ -98,6 +106,10 @@ struct DevGraph {
   const uint2* csr_rec;
   const std::uint32_t* csc_intra;
   const std::uint32_t* csr_intra;
+  // csr_bnd[v]: position in csr_rec of v's first boundary record; v's
+  // boundary records run to csr_bnd[v + 1], or to the end of the tile's CSR
+  // block for the tile's last vertex.
+  const std::uint32_t* csr_bnd;
 };
 
 // Control block; lives in device memory, one per workspace.
@@ -121,7 +133,25 @@ struct DevControl {
   unsigned long long algorithmic_bytes;  // modelled HBM bytes of the visits
   unsigned long long closure_sweeps;     // tile-closure sweeps (one per block pass)
   unsigned long long phase_cycles[kPhaseCount];  // summed over workgroups
+  // Arrival counters of the two-level grid barrier, one per XCD group of
+  // workgroups (blockIdx % 8), each on its own 128-byte line.
+  alignas(128) std::uint32_t group_arrivals[kBarrierGroups * kBarrierStride];
 };
+
+// What a visit of slot s needs before it can load any label: written once
+// per batch by the preparation kernel, so a visit starts with one 64-byte
+// read instead of a chain of dependent table reads.  The neighbour list and
+// edge block are those of the slot's engine: incoming tiles and the CSC
+// block for Jacobi pull, outgoing tiles and the CSR block for the push
+// engines.
+struct SlotDesc {
+  std::uint32_t tile, query, engine, v0;
+  std::uint32_t npt, nbr0, nbr_count, nbr_base;  // g.*_nbr_offsets[tile], list length, b.slot_*_base[s]
+  std::uint32_t e0, ei, e1, dist_lo;             // edge block [e0, ei) internal, [ei, e1) boundary
+  std::uint32_t dist_hi;                         // pool offset of the slot's labels (64 bits)
+  std::uint32_t out_nbr0, out_count, out_base;   // out-neighbour list and slots (whom to queue)
+};
+static_assert(sizeof(SlotDesc) == 64, "slot descriptor is one 64-byte record");
 
 struct DevBatch {
   std::uint32_t query_count;
@@ -146,8 +176,12 @@ struct DevBatch {
   const unsigned long long* slot_dist;  // [S]
   const std::uint32_t* slot_in_base;    // [S]
   const std::uint32_t* slot_out_base;   // [S]
+  const SlotDesc* slot_desc;            // [S] visit records, or null: derived per visit
   std::uint32_t* slot_in;
   std::uint32_t* slot_out;
+  // Pool offset of each out-neighbour slot (slot_out order), ~0 for none;
+  // null when visits look the offsets up through slot_dist.
+  unsigned long long* slot_out_off;
   std::uint32_t* stamp;                 // [S]
   std::uint32_t* lists;                 // [3*S]
   std::uint32_t* dist;                  // float bits pool
@@ -229,6 +263,96 @@ __device__ __forceinline__ void enqueue(const DevBatch& b, std::uint32_t slot,
   }
 }
 
+// Wave-wide enqueue: every lane of the wave calls it in uniform control
+// flow; lanes with `want` queue `slot` for `round`.  The stamps dedup per
+// slot as in enqueue(), and the lanes that queue a slot first share one
+// list-position claim per wave (one atomic on the list counter instead of
+// one per slot).
+__device__ __forceinline__ void enqueue_wave(const DevBatch& b, std::uint32_t slot, bool want,
+                                             std::uint32_t round) {
+  const std::uint32_t tag = round + 1;
+  const std::uint32_t lane = threadIdx.x & 63u;
+  const bool first = want && atomic_max_agent(&b.stamp[slot], tag) < tag;
+  const unsigned long long m = __ballot(first);
+  if (m == 0) return;
+  const std::uint32_t li = round % 3;
+  std::uint32_t base = 0;
+  if (lane == 0) base = atomic_add_agent(&b.control->list_count[li], static_cast<std::uint32_t>(__popcll(m)));
+  base = __shfl(base, 0);
+  if (first) {
+    const std::uint32_t rank = static_cast<std::uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
+    b.lists[static_cast<std::size_t>(li) * b.slot_count + base + rank] = slot;
+    if (b.pending) atomic_add_agent(&b.pending[b.slot_query[slot]], 1u);
+  }
+}
+
+// Neighbour entries of a visit's LDS table, and the words of LDS one
+// relaxation workgroup uses: own + orig labels, the work area, the table and
+// the flags.
+__host__ __device__ constexpr std::uint32_t lds_table_entries(std::uint32_t max_neighbor_tiles) {
+  return (max_neighbor_tiles + 3u) & ~3u;
+}
+__host__ __device__ constexpr std::size_t relax_lds_words(std::uint32_t max_v, std::uint32_t work_words,
+                                                          std::uint32_t max_neighbor_tiles) {
+  return 2ull * max_v + work_words + 4ull * lds_table_entries(max_neighbor_tiles) + kLdsFlagWords;
+}
+
+// Grid barriers of the persistent kernels (every workgroup of the grid
+// calls them the same number of times; the grid is co-resident).
+//
+// Flat: every workgroup arrives on one counter; the last arriver resets it
+// and advances the generation that the others wait on.
+__device__ __forceinline__ void grid_barrier_flat(DevControl* c, std::uint32_t blocks) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const std::uint32_t gen = load_agent(&c->barrier_generation);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const std::uint32_t arrived = atomic_add_agent(&c->barrier_count, 1u);
+    if (arrived == blocks - 1) {
+      store_agent(&c->barrier_count, 0u);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      store_agent(&c->barrier_generation, gen + 1);
+    } else {
+      while (load_agent(&c->barrier_generation) == gen) __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+// Two-level: a workgroup arrives on its group's counter (group =
+// blockIdx % 8, the XCD it was dispatched to); the last arriver of a group
+// resets that counter and arrives on the global one; the last group to
+// arrive resets the global counter and advances the generation.  Every
+// counter is reset before the generation moves, so the next barrier's
+// arrivals always find zeros.
+__device__ __forceinline__ void grid_barrier_grouped(DevControl* c, std::uint32_t blocks) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const std::uint32_t gen = load_agent(&c->barrier_generation);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const std::uint32_t g = blockIdx.x % kBarrierGroups;
+    const std::uint32_t members = (blocks + kBarrierGroups - 1u - g) / kBarrierGroups;
+    const std::uint32_t groups = blocks < kBarrierGroups ? blocks : kBarrierGroups;
+    std::uint32_t* mine = &c->group_arrivals[g * kBarrierStride];
+    bool last = false;
+    if (atomic_add_agent(mine, 1u) == members - 1) {
+      store_agent(mine, 0u);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      last = atomic_add_agent(&c->barrier_count, 1u) == groups - 1;
+    }
+    if (last) {
+      store_agent(&c->barrier_count, 0u);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      store_agent(&c->barrier_generation, gen + 1);
+    } else {
+      while (load_agent(&c->barrier_generation) == gen) __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
 // Position of key in the sorted range values[begin, end), or kNone.
 __device__ __forceinline__ std::uint32_t find_sorted(const std::uint32_t* values,
                                                      std::uint32_t begin, std::uint32_t end,
@@ -244,6 +368,38 @@ __device__ __forceinline__ std::uint32_t find_sorted(const std::uint32_t* values
 // Slot at position pos of a query's selected tiles.
 __device__ __forceinline__ std::uint32_t slot_at(const DevBatch& b, std::uint32_t pos) {
   return b.query_slot ? b.query_slot[pos] : pos;
+}
+
+// The visit record of slot s, derived from the batch and graph tables.
+__device__ __forceinline__ SlotDesc make_slot_desc(const DevGraph& g, const DevBatch& b,
+                                                   std::uint32_t s) {
+  SlotDesc d;
+  d.tile = b.tiles[s];
+  d.query = b.slot_query[s];
+  d.engine = b.engine[d.query];
+  d.v0 = g.tile_vertex_offsets[d.tile];
+  d.npt = g.tile_vertex_offsets[d.tile + 1] - d.v0;
+  const bool pull = d.engine == kJacobiPull;
+  const std::uint32_t* nbr = pull ? g.in_nbr_offsets : g.out_nbr_offsets;
+  d.nbr0 = nbr[d.tile];
+  d.nbr_count = nbr[d.tile + 1] - d.nbr0;
+  d.nbr_base = pull ? b.slot_in_base[s] : b.slot_out_base[s];
+  const std::uint32_t* off = pull ? g.column_offsets : g.row_offsets;
+  d.e0 = off[d.v0];
+  d.e1 = off[d.v0 + d.npt];
+  d.ei = d.e0 + (pull ? g.csc_intra : g.csr_intra)[d.tile];
+  const unsigned long long dist = b.slot_dist[s];
+  d.dist_lo = static_cast<std::uint32_t>(dist);
+  d.dist_hi = static_cast<std::uint32_t>(dist >> 32);
+  d.out_nbr0 = g.out_nbr_offsets[d.tile];
+  d.out_count = g.out_nbr_offsets[d.tile + 1] - d.out_nbr0;
+  d.out_base = b.slot_out_base[s];
+  return d;
+}
+
+__device__ __forceinline__ SlotDesc load_slot_desc(const DevGraph& g, const DevBatch& b,
+                                                   std::uint32_t s) {
+  return b.slot_desc ? b.slot_desc[s] : make_slot_desc(g, b, s);
 }
 
 // Slot of query q on tile, or kNone when q did not select the tile.

@@ -16,6 +16,23 @@ std::size_t device_lds_limit(bfn_device device);
 std::uint32_t device_cu_count(bfn_device device);
 // Copies the graph's device view and the largest in/out neighbour-tile list.
 void graph_view(bfn_graph graph, DevGraph* view, std::uint32_t* max_in, std::uint32_t* max_out);
+// What a batch entry point outside capi.hip needs of a resident graph: its
+// device view, host copies of the tile and neighbour-list offsets ([T+1]
+// each), its stream and its device.
+struct GraphHost {
+  DevGraph view;
+  const std::uint32_t* tile_offsets;
+  const std::uint32_t* in_offsets;
+  const std::uint32_t* out_offsets;
+  hipStream_t stream;
+  int ordinal;
+  std::uint32_t cu_count;
+  std::size_t lds_limit;
+};
+GraphHost graph_host(bfn_graph graph);
+// Checks of bfn_batch_submit's arguments, shared by every batch entry point.
+int32_t check_run_options(const bfn_run_options* options);
+int32_t check_batch(bfn_graph graph, const bfn_batch_desc* batch, const bfn_run_options* options);
 
 struct PathArgs;
 
@@ -57,11 +74,27 @@ struct TileImageArgs {
   std::uint32_t* csr_perm;
   std::uint32_t* csc_intra;
   std::uint32_t* csr_intra;
+  std::uint32_t* csr_bnd;     // [V] first boundary record of each vertex
   std::uint32_t* stats;
   std::uint32_t* host_in_total;
   std::uint32_t* host_out_total;
 };
 hipError_t derive_tile_image(const TileImageArgs& args, hipStream_t stream);
+// LDS words of the work area of a relaxation workgroup: Jacobi stages every
+// in-neighbour slab; push engines keep the snapshot and, in direct mode, the
+// boundary record starts and the list of pushing vertices; proposals mode
+// adds the out-neighbour proposal slabs after the snapshot.
+inline std::uint32_t relax_work_words(const DevGraph& g, bool any_jacobi, bool any_push,
+                                      bool proposals) {
+  std::uint32_t w = g.max_tile_vertices;
+  if (any_jacobi) w = w > g.max_stage_floats ? w : g.max_stage_floats;
+  if (any_push) {
+    const std::uint32_t push = proposals ? g.max_tile_vertices + g.max_out_stage_floats
+                                         : 3u * g.max_tile_vertices;
+    w = w > push ? w : push;
+  }
+  return w;
+}
 // Re-reads the record weights from the orientation weight arrays.
 hipError_t refresh_record_weights(const DevGraph& g, const std::uint32_t* csr_perm,
                                   const std::uint32_t* csc_perm, hipStream_t stream);

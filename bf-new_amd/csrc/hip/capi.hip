@@ -103,8 +103,8 @@ constexpr std::size_t align_up(std::size_t x, std::size_t a) { return (x + a - 1
 
 // LDS words of one relaxation workgroup: own + orig labels, the work area
 // (staged slabs, or snapshot + proposals), the table and the flags.
-constexpr std::size_t lds_words(std::uint32_t max_v, std::uint32_t work_words) {
-  return 2ull * max_v + work_words + kLdsTableWords + kLdsFlagWords;
+std::size_t lds_words(const DevGraph& g, std::uint32_t max_v, std::uint32_t work_words) {
+  return relax_lds_words(max_v, work_words, g.max_neighbor_tiles);
 }
 
 bool valid_block(std::uint32_t b) {
@@ -125,7 +125,7 @@ struct bfn_graph_s {
   std::uint32_t V = 0, E = 0, T = 0, Rcsr = 0, Rcsc = 0;
   DeviceBuffer owner, tile_off, row_off, dst, w, row_runs, run_edges, run_dst, col_off, src, cw,
       eid, col_runs, crun_edges, crun_src, in_off, out_off, csc_pack, csr_pack, csc_rec, csr_rec,
-      csc_perm, csr_perm, csc_intra, csr_intra, base_cost, occ_scratch, counts, stats;
+      csc_perm, csr_perm, csc_intra, csr_intra, csr_bnd, base_cost, occ_scratch, counts, stats;
   std::uint32_t* in_tiles = nullptr;
   std::uint32_t* in_stage = nullptr;
   std::uint32_t* out_tiles = nullptr;
@@ -160,7 +160,8 @@ struct bfn_workspace_s {
   hipStream_t stream = nullptr;
   std::uint64_t budget = 0;
   // batch metadata in one device buffer + separately grown state
-  DeviceBuffer meta, slot_in, slot_out, stamp, lists, pool, last_change, control, status, relax,
+  DeviceBuffer meta, slot_in, slot_out, slot_out_off, slot_desc, stamp, lists, pool, last_change, control,
+      status, relax,
       scratch_v, paths_summary, paths_len, paths_off, frames, onpath, clean, arena_v, arena_l,
       arena_e, occupancy, region_off;
   PinnedBuffer staging, h_status, h_control, h_paths;
@@ -402,6 +403,7 @@ int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_gra
   if (e == hipSuccess) e = g->csr_perm.reserve(sizeof(std::uint32_t) * e_words);
   if (e == hipSuccess) e = g->csc_intra.reserve(sizeof(std::uint32_t) * (T + 1));
   if (e == hipSuccess) e = g->csr_intra.reserve(sizeof(std::uint32_t) * (T + 1));
+  if (e == hipSuccess) e = g->csr_bnd.reserve(sizeof(std::uint32_t) * std::max(V, 1u));
   if (e == hipSuccess) e = g->stats.reserve(sizeof(std::uint32_t) * 8);
   if (e == hipSuccess) e = hipMemsetAsync(g->stats.ptr, 0, sizeof(std::uint32_t) * 8, s);
   if (e != hipSuccess) return cleanup(BFN_E_HIP, std::string("graph upload: ") + hipGetErrorString(e));
@@ -436,6 +438,7 @@ int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_gra
   ta.csr_perm = g->csr_perm.as<std::uint32_t>();
   ta.csc_intra = g->csc_intra.as<std::uint32_t>();
   ta.csr_intra = g->csr_intra.as<std::uint32_t>();
+  ta.csr_bnd = g->csr_bnd.as<std::uint32_t>();
   ta.stats = g->stats.as<std::uint32_t>();
   ta.host_in_total = &in_total;
   ta.host_out_total = &out_total;
@@ -465,6 +468,10 @@ int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_gra
   v.max_in_edges = stats[2];
   v.max_out_edges = stats[5];
   v.max_out_stage_floats = stats[6];
+  std::uint32_t max_nbr = 1;
+  for (std::uint32_t t = 0; t < T; ++t)
+    max_nbr = std::max({max_nbr, g->h_in_off[t + 1] - g->h_in_off[t], g->h_out_off[t + 1] - g->h_out_off[t]});
+  v.max_neighbor_tiles = max_nbr;
   v.owner_tiles = g->owner.as<std::uint32_t>();
   v.tile_vertex_offsets = g->tile_off.as<std::uint32_t>();
   v.row_offsets = g->row_off.as<std::uint32_t>();
@@ -486,6 +493,7 @@ int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_gra
   v.csr_rec = g->csr_rec.as<uint2>();
   v.csc_intra = g->csc_intra.as<std::uint32_t>();
   v.csr_intra = g->csr_intra.as<std::uint32_t>();
+  v.csr_bnd = g->csr_bnd.as<std::uint32_t>();
 
   bfn_graph_info& info = g->info;
   info.max_tile_vertices = max_v;
@@ -501,7 +509,7 @@ int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_gra
   info.has_zero_weight = stats[4];
   // reference image + derived tile lists + packs (4 B) + records (8 B) +
   // record permutations (4 B) per edge and orientation
-  info.device_bytes = 4ull * (2ull * V + 2 + T + 1 + 6ull * E + 2ull * (g->Rcsr + g->Rcsc + 2) +
+  info.device_bytes = 4ull * (3ull * V + 2 + T + 1 + 6ull * E + 2ull * (g->Rcsr + g->Rcsc + 2) +
                               4ull * (T + 1) + 2ull * in_total + 2ull * out_total) +
                       2ull * E * (4 + 8 + 4);
   *out = g;
@@ -630,9 +638,7 @@ void bfn_tuning_default(bfn_tuning* out) {
   out->count_relaxations = 0;
 }
 
-namespace {
-
-int32_t validate_options(const bfn_run_options* o) {
+int32_t bfn::check_run_options(const bfn_run_options* o) {
   if (!o) return fail(BFN_E_INVALID_ARGUMENT, "null run options");
   if (o->engine > 2) return fail(BFN_E_INVALID_ARGUMENT, "invalid engine");
   if (o->control_mode > 2) return fail(BFN_E_INVALID_ARGUMENT, "invalid control mode");
@@ -648,6 +654,10 @@ int32_t validate_options(const bfn_run_options* o) {
   return BFN_OK;
 }
 
+namespace {
+
+int32_t validate_options(const bfn_run_options* o) { return check_run_options(o); }
+
 struct BatchPlanHost {
   std::uint32_t Q = 0, S = 0;
   std::uint64_t pool_words = 0, frame_words = 0, in_total = 0, out_total = 0;
@@ -657,8 +667,11 @@ struct BatchPlanHost {
   bool any_push = false;
 };
 
-int32_t plan_batch(bfn_graph g, const bfn_batch_desc* d, const bfn_run_options* o,
-                   BatchPlanHost& p) {
+}  // namespace
+
+// Host checks of a batch description (shapes, canonical terminals inside
+// their query's selected tiles, sorted unique tiles in range, engines).
+int32_t bfn::check_batch(bfn_graph g, const bfn_batch_desc* d, const bfn_run_options* o) {
   if (!d) return fail(BFN_E_INVALID_ARGUMENT, "null batch");
   const std::uint32_t Q = d->query_count;
   if (Q == 0) return fail(BFN_E_INVALID_ARGUMENT, "empty batch");
@@ -700,6 +713,15 @@ int32_t plan_batch(bfn_graph g, const bfn_batch_desc* d, const bfn_run_options* 
   }
   if (d->source_offsets[0] != 0 || d->target_offsets[0] != 0 || d->tile_offsets[0] != 0)
     return fail(BFN_E_INVALID_ARGUMENT, "offsets must start at zero");
+  return BFN_OK;
+}
+
+namespace {
+
+int32_t plan_batch(bfn_graph g, const bfn_batch_desc* d, const bfn_run_options* o,
+                   BatchPlanHost& p) {
+  if (int32_t rc = check_batch(g, d, o)) return rc;
+  const std::uint32_t Q = d->query_count;
   p.Q = Q;
   p.S = d->tile_offsets[Q];
   const std::uint32_t S = p.S;
@@ -770,7 +792,8 @@ std::size_t meta_layout(std::uint32_t Q, std::uint32_t S, std::uint32_t ns, std:
 }
 
 std::uint64_t batch_bytes(const BatchPlanHost& p, std::uint64_t meta_bytes) {
-  return meta_bytes + 4ull * (p.in_total + p.out_total) + 16ull * p.S +
+  return meta_bytes + 4ull * (p.in_total + p.out_total) + 8ull * p.out_total +
+         (16ull + sizeof(SlotDesc)) * p.S +
          4ull * p.pool_words + 4ull * p.Q * 2 + sizeof(QueryStatusDev) * p.Q + 8ull * p.Q +
          sizeof(DevControl);
 }
@@ -804,6 +827,8 @@ void bind_view(bfn_workspace ws, const BatchPlanHost& p, const std::size_t* seg,
   b.slot_out_base = reinterpret_cast<const std::uint32_t*>(m + seg[10]);
   b.slot_in = ws->slot_in.as<std::uint32_t>();
   b.slot_out = ws->slot_out.as<std::uint32_t>();
+  b.slot_out_off = ws->slot_out_off.as<unsigned long long>();
+  b.slot_desc = ws->slot_desc.as<SlotDesc>();
   b.stamp = ws->stamp.as<std::uint32_t>();
   b.lists = ws->lists.as<std::uint32_t>();
   b.dist = ws->pool.as<std::uint32_t>();
@@ -988,12 +1013,10 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
   // Push engines stage their boundary proposals after the snapshot when the
   // LDS holds them; otherwise they push with direct global atomics.
   const bool proposals = p.any_push && t.boundary_mode == 0 &&
-                         lds_words(max_v, gv.max_tile_vertices + gv.max_out_stage_floats) * 4ull <=
+                         lds_words(gv, max_v, gv.max_tile_vertices + gv.max_out_stage_floats) * 4ull <=
                              lds_limit;
-  const std::uint32_t work_words =
-      std::max({p.any_jacobi ? gv.max_stage_floats : 0u, gv.max_tile_vertices,
-                proposals ? gv.max_tile_vertices + gv.max_out_stage_floats : 0u});
-  const std::size_t lds_bytes = 4ull * lds_words(max_v, work_words);
+  const std::uint32_t work_words = relax_work_words(gv, p.any_jacobi, p.any_push, proposals);
+  const std::size_t lds_bytes = 4ull * lds_words(gv, max_v, work_words);
   if (lds_bytes > lds_limit)
     return fail(BFN_E_NOT_REPRESENTABLE, "tile neighbourhood exceeds the LDS budget (" +
                                              std::to_string(lds_bytes) + " > " +
@@ -1026,6 +1049,8 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
   BFN_HIP(ws->meta.reserve(meta_bytes));
   BFN_HIP(ws->slot_in.reserve(4ull * std::max<std::uint64_t>(p.in_total, 1)));
   BFN_HIP(ws->slot_out.reserve(4ull * std::max<std::uint64_t>(p.out_total, 1)));
+  BFN_HIP(ws->slot_out_off.reserve(8ull * std::max<std::uint64_t>(p.out_total, 1)));
+  BFN_HIP(ws->slot_desc.reserve(sizeof(SlotDesc) * std::max(S, 1u)));
   BFN_HIP(ws->stamp.reserve(4ull * std::max(S, 1u)));
   BFN_HIP(ws->lists.reserve(12ull * std::max(S, 1u)));
   BFN_HIP(ws->pool.reserve(4ull * std::max<std::uint64_t>(p.pool_words, 1)));
@@ -1331,5 +1356,17 @@ void graph_view(bfn_graph graph, DevGraph* view, std::uint32_t* max_in, std::uin
   *view = graph->view;
   *max_in = graph->info.max_in_neighbor_tiles;
   *max_out = graph->info.max_out_neighbor_tiles;
+}
+GraphHost graph_host(bfn_graph graph) {
+  GraphHost h;
+  h.view = graph->view;
+  h.tile_offsets = graph->h_tile_off.data();
+  h.in_offsets = graph->h_in_off.data();
+  h.out_offsets = graph->h_out_off.data();
+  h.stream = graph->stream;
+  h.ordinal = graph->device->ordinal;
+  h.cu_count = static_cast<std::uint32_t>(graph->device->props.multiProcessorCount);
+  h.lds_limit = graph->device->props.sharedMemPerBlock;
+  return h;
 }
 }  // namespace bfn

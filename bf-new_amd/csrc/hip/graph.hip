@@ -214,6 +214,38 @@ __global__ __launch_bounds__(256) void split_records_kernel(SplitArgs a) {
   if (tid == 0) a.intra[T] = n_int;
 }
 
+// Per vertex, the position of its first boundary record in the tile's
+// [internal | boundary] block: the boundary part keeps orientation order, so
+// a vertex's boundary records follow those of the tile's earlier vertices.
+// One workgroup per tile; a carry runs across 256-vertex chunks.
+__global__ __launch_bounds__(256) void boundary_starts_kernel(const std::uint32_t* tile_vertex_offsets,
+                                                              const std::uint32_t* offsets,
+                                                              const std::uint32_t* pack,
+                                                              const std::uint32_t* intra,
+                                                              std::uint32_t* starts) {
+  __shared__ std::uint32_t buf[256];
+  const std::uint32_t T = blockIdx.x, tid = threadIdx.x;
+  const std::uint32_t v0 = tile_vertex_offsets[T], v1 = tile_vertex_offsets[T + 1];
+  std::uint32_t carry = offsets[v0] + intra[T];
+  for (std::uint32_t base = v0; base < v1; base += 256) {
+    const std::uint32_t v = base + tid;
+    std::uint32_t count = 0;
+    if (v < v1)
+      for (std::uint32_t e = offsets[v]; e < offsets[v + 1]; ++e) count += (pack[e] >> 24) != 0u;
+    buf[tid] = count;
+    __syncthreads();
+    for (std::uint32_t off = 1; off < 256; off <<= 1) {
+      const std::uint32_t y = tid >= off ? buf[tid - off] : 0u;
+      __syncthreads();
+      buf[tid] += y;
+      __syncthreads();
+    }
+    if (v < v1) starts[v] = carry + buf[tid] - count;
+    carry += buf[255];
+    __syncthreads();
+  }
+}
+
 __global__ void refresh_weights_kernel(uint2* rec, const std::uint32_t* perm, const float* w,
                                        std::uint32_t n) {
   const std::uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -295,6 +327,8 @@ hipError_t derive_tile_image(const TileImageArgs& t, hipStream_t stream) {
                        t.csr_rec, t.csr_perm, t.csr_intra};
   hipLaunchKernelGGL(split_records_kernel, dim3(t.tile_count), dim3(256), 0, stream, sin);
   hipLaunchKernelGGL(split_records_kernel, dim3(t.tile_count), dim3(256), 0, stream, sout);
+  hipLaunchKernelGGL(boundary_starts_kernel, dim3(t.tile_count), dim3(256), 0, stream,
+                     t.tile_vertex_offsets, t.row_offsets, t.csr_pack, t.csr_intra, t.csr_bnd);
   return hipGetLastError();
 }
 

@@ -41,22 +41,35 @@ struct Lds {
   std::uint32_t* own;
   std::uint32_t* orig;
   std::uint32_t* work;
-  std::uint32_t* prop;
-  std::uint32_t* table;
+  std::uint32_t* prop;   // push, proposals: the boundary proposal slabs
+  std::uint32_t* bnd;    // push, direct: boundary record start per vertex
+  std::uint32_t* list;   // push, direct: the visit's pushing vertices
+  std::uint32_t* slots;  // per neighbour: its slot (push) / staged slab offset (pull)
+  std::uint32_t* marks;  // per neighbour: pushed into during this visit
+  std::uint32_t* offs;   // per neighbour: pool offset (two words) / proposal slab offset
   std::uint32_t* flags;
 };
 
 __device__ __forceinline__ Lds carve(std::uint32_t* base, std::uint32_t max_v,
-                                     std::uint32_t work_words) {
+                                     std::uint32_t work_words, std::uint32_t max_neighbor_tiles) {
+  const std::uint32_t k = lds_table_entries(max_neighbor_tiles);
   Lds l;
   l.own = base;
   l.orig = base + max_v;
   l.work = base + 2 * max_v;
   l.prop = l.work + max_v;
-  l.table = l.work + work_words;
-  l.flags = l.table + kLdsTableWords;
+  l.bnd = l.work + max_v;
+  l.list = l.work + 2 * max_v;
+  l.slots = l.work + work_words;
+  l.marks = l.slots + k;
+  l.offs = l.marks + k;
+  l.flags = l.offs + 2 * k;
   return l;
 }
+
+// Boundary records per pushing vertex loaded together (one load, then one
+// filter read, in flight per record).
+constexpr int kPushChunk = 8;
 
 // Edge records in registers: thread t holds records t, t + B, t + 2B, ... of
 // the range it loaded.
@@ -122,28 +135,28 @@ __device__ __forceinline__ std::uint32_t pk_dst(std::uint32_t p) { return p & kP
 // Jacobi pull visit.
 // ---------------------------------------------------------------------------
 template <int B, int EPT, bool I>
-__device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b, std::uint32_t s,
+__device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b, const SlotDesc& d,
                            std::uint32_t r, const Lds& l, RecordRegs<B, EPT>& internal,
                            std::uint32_t& regs_key, VisitStats<I>& st) {
   const std::uint32_t tid = threadIdx.x;
   st.start();
-  const std::uint32_t T = b.tiles[s];
-  const std::uint32_t q = b.slot_query[s];
-  const std::uint32_t v0 = g.tile_vertex_offsets[T];
-  const std::uint32_t npt = g.tile_vertex_offsets[T + 1] - v0;
-  const std::uint32_t nb0 = g.in_nbr_offsets[T];
-  const std::uint32_t K = g.in_nbr_offsets[T + 1] - nb0;
+  const std::uint32_t T = d.tile;
+  const std::uint32_t q = d.query;
+  const std::uint32_t npt = d.npt;
+  const std::uint32_t nb0 = d.nbr0;
+  const std::uint32_t K = d.nbr_count;
   const std::uint32_t rp = r & 1u;
-  const std::uint32_t in_base = b.slot_in_base[s];
+  const std::uint32_t in_base = d.nbr_base;
+  const unsigned long long dist = (static_cast<unsigned long long>(d.dist_hi) << 32) | d.dist_lo;
 
-  for (std::uint32_t k = tid; k < K; k += B) l.table[k] = g.in_nbr_stage[nb0 + k];
+  for (std::uint32_t k = tid; k < K; k += B) l.slots[k] = g.in_nbr_stage[nb0 + k];
   __syncthreads();
   // Stage the read column of every in-neighbour slab (k = 0 is T itself).
   for (std::uint32_t k = 0; k < K; ++k) {
     const std::uint32_t nt = g.in_nbr_tiles[nb0 + k];
     const std::uint32_t n = g.tile_vertex_offsets[nt + 1] - g.tile_vertex_offsets[nt];
     const std::uint32_t ns = b.slot_in[in_base + k];
-    std::uint32_t* dst = l.work + l.table[k];
+    std::uint32_t* dst = l.work + l.slots[k];
     if (ns == kNone) {
       for (std::uint32_t i = tid; i < n; i += B) dst[i] = kInfBits;
     } else {
@@ -158,9 +171,9 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
   st.mark(0);
 
   // CSC block of T: internal records [e0, ei), boundary records [ei, e1).
-  const std::uint32_t e0 = g.column_offsets[v0];
-  const std::uint32_t e1 = g.column_offsets[v0 + npt];
-  const std::uint32_t ei = e0 + g.csc_intra[T];
+  const std::uint32_t e0 = d.e0;
+  const std::uint32_t e1 = d.e1;
+  const std::uint32_t ei = d.ei;
   const bool resident = ei - e0 <= static_cast<std::uint32_t>(B * EPT);
   if (tid == 0) {
     // edge records (pack + weight) once, the written column once
@@ -169,7 +182,7 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
   }
   // Pass 1: every admitted in-edge reads the staged read column.
   auto relax_staged = [&](std::uint32_t p, float w) {
-    const std::uint32_t cb = as_bits(as_float(l.work[l.table[pk_k(p)] + pk_src(p)]) + w);
+    const std::uint32_t cb = as_bits(as_float(l.work[l.slots[pk_k(p)] + pk_src(p)]) + w);
     if (cb < l.own[pk_dst(p)]) atomicMin(&l.own[pk_dst(p)], cb);
   };
   // The internal records stay in registers from a previous visit of T.
@@ -223,7 +236,7 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
   }
   st.mark(2);
 
-  std::uint32_t* out = b.dist + b.slot_dist[s] + static_cast<std::size_t>(rp ^ 1u) * npt;
+  std::uint32_t* out = b.dist + dist + static_cast<std::size_t>(rp ^ 1u) * npt;
   bool changed = false;
   for (std::uint32_t i = tid; i < npt; i += B) {
     const std::uint32_t v = l.own[i];
@@ -232,12 +245,12 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
   }
   if (__syncthreads_or(changed)) {
     if (tid == 0) atomic_max_agent(&b.last_change[q], r + 1);
-    const std::uint32_t ob0 = g.out_nbr_offsets[T];
-    const std::uint32_t KO = g.out_nbr_offsets[T + 1] - ob0;
-    const std::uint32_t out_base = b.slot_out_base[s];
-    for (std::uint32_t j = tid; j < KO; j += B) {
-      const std::uint32_t ns = b.slot_out[out_base + j];
-      if (ns != kNone) enqueue(b, ns, r + 1);
+    // every selected out-neighbour (the tile itself first) reads the new
+    // column next round
+    for (std::uint32_t j0 = 0; j0 < d.out_count; j0 += B) {
+      const std::uint32_t j = j0 + tid;
+      const std::uint32_t ns = j < d.out_count ? b.slot_out[d.out_base + j] : kNone;
+      enqueue_wave(b, ns == kNone ? 0u : ns, ns != kNone, r + 1);
     }
   }
   st.mark(4);
@@ -247,34 +260,35 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
 // Dense / frontier push visit.
 // ---------------------------------------------------------------------------
 template <int B, int EPT, bool I>
-__device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b, std::uint32_t s,
+__device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b, const SlotDesc& d,
                            std::uint32_t r, bool frontier, const Lds& l,
                            RecordRegs<B, EPT>& internal, std::uint32_t& regs_key,
                            VisitStats<I>& st) {
   const std::uint32_t tid = threadIdx.x;
   st.start();
-  const std::uint32_t T = b.tiles[s];
-  const std::uint32_t q = b.slot_query[s];
-  const std::uint32_t v0 = g.tile_vertex_offsets[T];
-  const std::uint32_t npt = g.tile_vertex_offsets[T + 1] - v0;
-  const std::uint32_t ob0 = g.out_nbr_offsets[T];
-  const std::uint32_t KO = g.out_nbr_offsets[T + 1] - ob0;
-  const std::uint32_t out_base = b.slot_out_base[s];
-  std::uint32_t* column = b.dist + b.slot_dist[s];
+  const std::uint32_t T = d.tile;
+  const std::uint32_t q = d.query;
+  const std::uint32_t v0 = d.v0;
+  const std::uint32_t npt = d.npt;
+  const std::uint32_t ob0 = d.nbr0;
+  const std::uint32_t KO = d.nbr_count;
+  const std::uint32_t out_base = d.nbr_base;
+  std::uint32_t* column = b.dist + ((static_cast<unsigned long long>(d.dist_hi) << 32) | d.dist_lo);
   std::uint32_t* snapshot = column + npt;
 
   const bool proposals = b.boundary_mode == 0;
   for (std::uint32_t k = tid; k < KO; k += B) {
     const std::uint32_t ns = b.slot_out[out_base + k];
-    l.table[k] = ns;
-    l.table[256 + k] = 0;
+    l.slots[k] = ns;
+    l.marks[k] = 0;
     if (proposals) {
-      l.table[512 + k] = g.out_nbr_stage[ob0 + k];
-    } else if (ns != kNone) {
+      l.offs[k] = g.out_nbr_stage[ob0 + k];
+    } else {
       // pool offset of the neighbour's labels, as two words
-      const unsigned long long off = b.slot_dist[ns];
-      l.table[512 + 2 * k] = static_cast<std::uint32_t>(off);
-      l.table[513 + 2 * k] = static_cast<std::uint32_t>(off >> 32);
+      const unsigned long long off =
+          b.slot_out_off ? b.slot_out_off[out_base + k] : (ns != kNone ? b.slot_dist[ns] : ~0ull);
+      l.offs[2 * k] = static_cast<std::uint32_t>(off);
+      l.offs[2 * k + 1] = static_cast<std::uint32_t>(off >> 32);
     }
   }
   for (std::uint32_t i = tid; i < npt; i += B) {
@@ -283,7 +297,9 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
     l.orig[i] = x;
     // Dense pushes every finite vertex: an infinite snapshot admits them all.
     l.work[i] = frontier ? snapshot[i] : kInfBits;
+    if (!proposals) l.bnd[i] = g.csr_bnd[v0 + i];
   }
+  if (tid == 0) l.flags[1] = 0;
   if (proposals) {
     // Clear the proposal slab of every selected out-neighbour (k >= 1).
     for (std::uint32_t k = 1; k < KO; ++k) {
@@ -298,14 +314,17 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
   st.mark(0);
 
   // CSR block of T: internal records [e0, ei), boundary records [ei, e1).
-  const std::uint32_t e0 = g.row_offsets[v0];
-  const std::uint32_t e1 = g.row_offsets[v0 + npt];
-  const std::uint32_t ei = e0 + g.csr_intra[T];
+  const std::uint32_t e0 = d.e0;
+  const std::uint32_t e1 = d.e1;
+  const std::uint32_t ei = d.ei;
   const bool resident = ei - e0 <= static_cast<std::uint32_t>(B * EPT);
   if (tid == 0) {
-    // edge records once, the column (and the snapshot for frontier) once
-    st.bytes += 8ull * (e1 - e0) + 4ull * npt * (frontier ? 2 : 1);
-    st.edges += e1 - e0;
+    // internal records once, the column (and the snapshot for frontier)
+    // once; proposals stream every boundary record, direct mode reads the
+    // record starts and then only the pushing vertices' records
+    st.bytes += 8ull * (ei - e0) + 4ull * npt * (frontier ? 2 : 1) +
+                (proposals ? 8ull * (e1 - ei) : 4ull * npt);
+    st.edges += (ei - e0) + (proposals ? e1 - ei : 0u);
   }
   // The internal records stay in registers from a previous visit of T.
   const std::uint32_t key = 2 * T + 1;
@@ -351,20 +370,20 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
         if (p == kNone) continue;
         const std::uint32_t sv = l.own[pk_src(p)];
         if (!(sv < l.work[pk_src(p)])) continue;
-        if (l.table[pk_k(p)] == kNone) continue;
+        if (l.slots[pk_k(p)] == kNone) continue;
         const std::uint32_t cb = as_bits(as_float(sv) + chunk.w[j]);
-        std::uint32_t* slot = &l.prop[l.table[512 + pk_k(p)] + pk_dst(p)];
+        std::uint32_t* slot = &l.prop[l.offs[pk_k(p)] + pk_dst(p)];
         ++att;
         if (cb < *slot) atomicMin(slot, cb);
       }
     }
     __syncthreads();
     for (std::uint32_t k = 1; k < KO; ++k) {
-      const std::uint32_t ns = l.table[k];
+      const std::uint32_t ns = l.slots[k];
       if (ns == kNone) continue;
       const std::uint32_t nt = g.out_nbr_tiles[ob0 + k];
       const std::uint32_t n = g.tile_vertex_offsets[nt + 1] - g.tile_vertex_offsets[nt];
-      const std::uint32_t* slab = l.prop + l.table[512 + k];
+      const std::uint32_t* slab = l.prop + l.offs[k];
       std::uint32_t* target = b.dist + b.slot_dist[ns];
       if (tid == 0) st.bytes += 4ull * n;  // the neighbour slab, read once
       bool hit = false;
@@ -376,43 +395,77 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
           hit = true;
         }
       }
-      if (hit) l.table[256 + k] = 1;
+      if (hit) l.marks[k] = 1;
     }
   } else {
-    // One global atomic per boundary candidate that beats the neighbour's
-    // label as read.  Per chunk the reads are issued together, then the
-    // atomics, which return nothing: a neighbour is marked when the read
-    // label was above the candidate, which covers every actual decrease.
-    auto candidate = [&](std::uint32_t p, float w, std::uint32_t*& target) -> std::uint32_t {
-      if (p == kNone) return kNone;
-      const std::uint32_t sv = l.own[pk_src(p)];
-      if (!(sv < l.work[pk_src(p)]) || l.table[pk_k(p)] == kNone) return kNone;
-      const unsigned long long off = (static_cast<unsigned long long>(l.table[513 + 2 * pk_k(p)]) << 32) |
-                                     l.table[512 + 2 * pk_k(p)];
-      target = b.dist + off + pk_dst(p);
-      return as_bits(as_float(sv) + w);
-    };
-    for (std::uint32_t base = ei; base < e1; base += B * stream_records(B)) {
-      RecordRegs<B, stream_records(B)> chunk;
-      load_records<B, stream_records(B)>(chunk, g.csr_rec, base, e1);
-      std::uint32_t seen[stream_records(B)];
+    // Direct mode: only vertices that push (label below the snapshot;
+    // dense: finite) read their boundary records.  They are listed in LDS
+    // (one claim per wave) and taken one vertex per thread; a vertex's
+    // boundary records are consecutive, loaded kPushChunk at a time, then
+    // their filter reads are issued together, then one no-return atomic per
+    // candidate that beats the label read.  The filter reads are ordinary
+    // (cacheable, possibly stale) loads: labels only fall, so a stale label
+    // is never below the true one and the filter never drops a decrease; a
+    // neighbour is marked when its read label was above a candidate, which
+    // covers every actual decrease.
+    for (std::uint32_t base = 0; base < npt; base += B) {
+      const std::uint32_t i = base + tid;
+      const bool push = i < npt && l.own[i] < l.work[i];
+      const unsigned long long m = __ballot(push);
+      std::uint32_t at = 0;
+      if ((tid & 63u) == 0 && m) at = atomicAdd(&l.flags[1], static_cast<std::uint32_t>(__popcll(m)));
+      at = __shfl(at, 0);
+      if (push) l.list[at + __popcll(m & ((1ull << (tid & 63u)) - 1ull))] = i;
+    }
+    __syncthreads();
+    const std::uint32_t n_push = l.flags[1];
+    unsigned long long loaded = 0;
+    for (std::uint32_t j = tid; j < n_push; j += B) {
+      const std::uint32_t i = l.list[j];
+      const float sf = as_float(l.own[i]);
+      const std::uint32_t rb = l.bnd[i];
+      const std::uint32_t re = i + 1 < npt ? l.bnd[i + 1] : e1;
+      loaded += re - rb;
+      for (std::uint32_t r0 = rb; r0 < re; r0 += kPushChunk) {
+        std::uint32_t pk[kPushChunk];
+        float w[kPushChunk];
 #pragma unroll
-      for (int j = 0; j < stream_records(B); ++j) {
-        std::uint32_t* target = nullptr;
-        seen[j] = candidate(chunk.pack[j], chunk.w[j], target) == kNone ? 0u : *target;
-      }
+        for (int c = 0; c < kPushChunk; ++c) {
+          pk[c] = kNone;
+          w[c] = 0.0f;
+          if (r0 + c < re) {
+            const uint2 x = g.csr_rec[r0 + c];
+            pk[c] = x.x;
+            w[c] = as_float(x.y);
+          }
+        }
+        auto target = [&](std::uint32_t p) -> std::uint32_t* {
+          const unsigned long long off = (static_cast<unsigned long long>(l.offs[2 * pk_k(p) + 1]) << 32) |
+                                         l.offs[2 * pk_k(p)];
+          return b.dist + off + pk_dst(p);
+        };
+        std::uint32_t seen[kPushChunk];
 #pragma unroll
-      for (int j = 0; j < stream_records(B); ++j) {
-        std::uint32_t* target = nullptr;
-        const std::uint32_t cb = candidate(chunk.pack[j], chunk.w[j], target);
-        if (cb == kNone) continue;
-        ++att;
-        if (cb < seen[j]) {
-          atomic_min_agent(target, cb);
-          ++suc;
-          l.table[256 + pk_k(chunk.pack[j])] = 1;
+        for (int c = 0; c < kPushChunk; ++c) {
+          if (pk[c] != kNone && l.slots[pk_k(pk[c])] == kNone) pk[c] = kNone;  // not selected
+          seen[c] = pk[c] != kNone ? *target(pk[c]) : 0u;
+        }
+#pragma unroll
+        for (int c = 0; c < kPushChunk; ++c) {
+          if (pk[c] == kNone) continue;
+          ++att;
+          const std::uint32_t cb = as_bits(sf + w[c]);
+          if (cb < seen[c]) {
+            atomic_min_agent(target(pk[c]), cb);
+            ++suc;
+            l.marks[pk_k(pk[c])] = 1;
+          }
         }
       }
+    }
+    if (loaded) {
+      st.edges += loaded;  // per-thread counters, summed at the end of the round
+      st.bytes += 8ull * loaded;
     }
   }
   st.attempts += att;
@@ -432,11 +485,11 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
   }
   const bool pushed_any = __syncthreads_or(changed);
   bool marked = false;
-  for (std::uint32_t k = 1 + tid; k < KO; k += B) {
-    if (l.table[256 + k]) {
-      enqueue(b, l.table[k], r + 1);
-      marked = true;
-    }
+  for (std::uint32_t k0 = 1; k0 < KO; k0 += B) {
+    const std::uint32_t k = k0 + tid;
+    const bool want = k < KO && l.marks[k] != 0u;
+    marked |= want;
+    enqueue_wave(b, want ? l.slots[k] : 0u, want, r + 1);
   }
   if (__syncthreads_or(marked) || pushed_any) {
     if (tid == 0) atomic_max_agent(&b.last_change[q], r + 1);
@@ -464,50 +517,34 @@ __device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch&
     const std::uint32_t end = min(start + chunk, n);
     for (std::uint32_t i = start; i < end; ++i) {
       const std::uint32_t s = b.lists[static_cast<std::size_t>(li) * b.slot_count + i];
-      const std::uint32_t engine = b.engine[b.slot_query[s]];
-      if (engine == kJacobiPull) {
-        visit_pull<B, EPT, I>(g, b, s, r, l, regs, regs_key, st);
+      const SlotDesc d = load_slot_desc(g, b, s);
+      if (d.engine == kJacobiPull) {
+        visit_pull<B, EPT, I>(g, b, d, r, l, regs, regs_key, st);
       } else {
-        visit_push<B, EPT, I>(g, b, s, r, engine == kFrontierPush, l, regs, regs_key, st);
+        visit_push<B, EPT, I>(g, b, d, r, d.engine == kFrontierPush, l, regs, regs_key, st);
       }
       ++st.items;
       __syncthreads();
       // the visit's own enqueues are issued before its item is retired
-      if (b.pending && threadIdx.x == 0) atomic_add_agent(&b.pending[b.slot_query[s]], 0xFFFFFFFFu);
+      if (b.pending && threadIdx.x == 0) atomic_add_agent(&b.pending[d.query], 0xFFFFFFFFu);
     }
   }
   if constexpr (I) {
     // One add per thread keeps the counters off the hot path's critical lanes.
     DevControl* c = b.control;
+    // Byte and record counts are per thread: tile-level terms are added by
+    // thread 0 of the visit, per-record terms by the thread that did them.
     if (st.attempts) atomicAdd(&c->atomic_attempts, st.attempts);
     if (st.successes) atomicAdd(&c->atomic_successes, st.successes);
+    if (st.bytes) atomicAdd(&c->algorithmic_bytes, st.bytes);
+    if (st.edges) atomicAdd(&c->edges_relaxed, st.edges);
     if (threadIdx.x == 0) {
       if (st.items) atomicAdd(&c->items_processed, st.items);
-      if (st.bytes) atomicAdd(&c->algorithmic_bytes, st.bytes);
-      if (st.edges) atomicAdd(&c->edges_relaxed, st.edges);
       if (st.sweeps) atomicAdd(&c->closure_sweeps, st.sweeps);
       for (int p = 0; p < kPhaseCount; ++p)
         if (st.phase[p]) atomicAdd(&c->phase_cycles[p], st.phase[p]);
     }
   }
-}
-
-__device__ __forceinline__ void grid_barrier(DevControl* c, std::uint32_t blocks) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const std::uint32_t gen = load_agent(&c->barrier_generation);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    const std::uint32_t arrived = atomic_add_agent(&c->barrier_count, 1u);
-    if (arrived == blocks - 1) {
-      store_agent(&c->barrier_count, 0u);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      store_agent(&c->barrier_generation, gen + 1);
-    } else {
-      while (load_agent(&c->barrier_generation) == gen) __builtin_amdgcn_s_sleep(1);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  }
-  __syncthreads();
 }
 
 // One cooperative launch runs every round: process list r%3, grid barrier,
@@ -517,7 +554,7 @@ __global__ __launch_bounds__(B, MW) void relax_persistent_kernel(DevGraph g, Dev
                                                              std::uint32_t max_v,
                                                              std::uint32_t work_words) {
   extern __shared__ std::uint32_t lds_words[];
-  const Lds l = carve(lds_words, max_v, work_words);
+  const Lds l = carve(lds_words, max_v, work_words, g.max_neighbor_tiles);
   DevControl* c = b.control;
   std::uint32_t r = load_agent(&c->round);
   if (load_agent(&c->done)) return;
@@ -531,7 +568,7 @@ __global__ __launch_bounds__(B, MW) void relax_persistent_kernel(DevGraph g, Dev
       if (b.instrumentation) atomicMax(&c->max_list, static_cast<unsigned long long>(n));
     }
     process_round<B, EPT, I>(g, b, r, n, l);
-    grid_barrier(c, gridDim.x);
+    grid_barrier_grouped(c, gridDim.x);
     ++r;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -550,7 +587,7 @@ __global__ __launch_bounds__(B, MW) void relax_round_kernel(DevGraph g, DevBatch
                                                         std::uint32_t max_v,
                                                         std::uint32_t work_words) {
   extern __shared__ std::uint32_t lds_words[];
-  const Lds l = carve(lds_words, max_v, work_words);
+  const Lds l = carve(lds_words, max_v, work_words, g.max_neighbor_tiles);
   DevControl* c = b.control;
   if (load_agent(&c->done)) return;
   const std::uint32_t r = load_agent(&c->round);
@@ -602,6 +639,7 @@ __global__ void reset_control_kernel(DevControl* c) {
     c->algorithmic_bytes = 0;
     c->closure_sweeps = 0;
     for (int p = 0; p < kPhaseCount; ++p) c->phase_cycles[p] = 0;
+    for (std::uint32_t i = 0; i < kBarrierGroups * kBarrierStride; ++i) c->group_arrivals[i] = 0;
   }
 }
 
@@ -624,9 +662,14 @@ __global__ void prep_slots_kernel(DevGraph g, DevBatch b) {
   for (std::uint32_t k = in0; k < in1; ++k) b.slot_in[ib + (k - in0)] = find_slot(b, q, g.in_nbr_tiles[k]);
   const std::uint32_t o0 = g.out_nbr_offsets[T], o1 = g.out_nbr_offsets[T + 1];
   const std::uint32_t obs = b.slot_out_base[s];
-  for (std::uint32_t k = o0; k < o1; ++k) b.slot_out[obs + (k - o0)] = find_slot(b, q, g.out_nbr_tiles[k]);
+  for (std::uint32_t k = o0; k < o1; ++k) {
+    const std::uint32_t ns = find_slot(b, q, g.out_nbr_tiles[k]);
+    b.slot_out[obs + (k - o0)] = ns;
+    if (b.slot_out_off) b.slot_out_off[obs + (k - o0)] = ns == kNone ? ~0ull : b.slot_dist[ns];
+  }
   b.stamp[s] = 0;
   b.last_change[q] = 0;
+  if (b.slot_desc) const_cast<SlotDesc*>(b.slot_desc)[s] = make_slot_desc(g, b, s);
 }
 
 // Zero the sources and queue round 0: Jacobi queues the source tiles and

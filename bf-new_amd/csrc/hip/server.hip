@@ -201,6 +201,13 @@ struct Active {
   double submit_ns;
 };
 
+// A response with its compact path (source first; empty when unreached).
+struct Delivered {
+  bfn_response response;
+  std::vector<std::uint32_t> vertices;
+  std::vector<std::uint32_t> edges;
+};
+
 }  // namespace
 
 struct bfn_server_s {
@@ -223,7 +230,13 @@ struct bfn_server_s {
                 *d_last = nullptr, *d_pending = nullptr, *d_active = nullptr, *d_done = nullptr,
                 *d_result = nullptr, *d_clean = nullptr, *d_summary = nullptr,
                 *d_upload_tiles = nullptr;
-  unsigned long long *d_slot_dist = nullptr, *d_frame_base = nullptr, *d_lengths = nullptr;
+  unsigned long long *d_slot_dist = nullptr, *d_frame_base = nullptr, *d_lengths = nullptr,
+                     *d_path_offsets = nullptr;
+  // Compact-path arenas of one step's converged queries (grown on demand).
+  std::uint32_t *d_arena_v = nullptr, *d_arena_l = nullptr, *d_arena_e = nullptr;
+  std::uint32_t *h_arena_v = nullptr, *h_arena_e = nullptr;
+  unsigned long long* h_path_offsets = nullptr;
+  std::uint64_t arena_capacity = 0;
   DevControl* d_control = nullptr;
   AdmitRecord* d_records = nullptr;
   // pinned staging
@@ -238,7 +251,7 @@ struct bfn_server_s {
   std::vector<std::uint32_t> free_queries;
   std::vector<Active> active;
   std::deque<Pending> inbox;
-  std::deque<bfn_response> outbox;
+  std::deque<Delivered> outbox;
   std::mutex mutex;
   std::condition_variable wake, delivered;
   bool stopping = false;
@@ -276,7 +289,35 @@ struct bfn_server_s {
   }
   void run();
   hipError_t step(std::uint32_t admit_count, std::uint32_t tile_count);
+  hipError_t reserve_arena(std::uint64_t vertices);
+  void free_arena();
 };
+
+void bfn_server_s::free_arena() {
+  for (std::uint32_t* p : {d_arena_v, d_arena_l, d_arena_e})
+    if (p) (void)hipFree(p);
+  for (std::uint32_t* p : {h_arena_v, h_arena_e})
+    if (p) (void)hipHostFree(p);
+  d_arena_v = d_arena_l = d_arena_e = h_arena_v = h_arena_e = nullptr;
+  arena_capacity = 0;
+}
+
+hipError_t bfn_server_s::reserve_arena(std::uint64_t vertices) {
+  if (vertices <= arena_capacity) return hipSuccess;
+  free_arena();
+  const std::uint64_t n = std::max<std::uint64_t>(vertices, 2 * static_cast<std::uint64_t>(max_v));
+  hipError_t e = hipSuccess;
+  for (std::uint32_t** p : {&d_arena_v, &d_arena_l, &d_arena_e})
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(p), 4 * n);
+  for (std::uint32_t** p : {&h_arena_v, &h_arena_e})
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(p), 4 * n, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    free_arena();
+    return e;
+  }
+  arena_capacity = n;
+  return hipSuccess;
+}
 
 hipError_t bfn_server_s::step(std::uint32_t admit_count, std::uint32_t tile_count) {
   hipStream_t s = stream;
@@ -332,6 +373,32 @@ hipError_t bfn_server_s::step(std::uint32_t admit_count, std::uint32_t tile_coun
         hipSuccess)
       return e;
     if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    // Second pass: the complete paths themselves, packed at host-computed
+    // offsets, then copied back with the summaries' step.
+    std::uint64_t total = 0;
+    for (std::uint32_t i = 0; i < ndone; ++i) {
+      const std::uint32_t q = h_done[1 + i];
+      const bool complete = h_summary[7ull * q + 5] == 0u && h_summary[7ull * q + 6] == 1u;
+      h_path_offsets[q] = total;
+      if (complete) total += h_summary[7ull * q + 3] + 1ull;
+    }
+    if (total) {
+      if ((e = reserve_arena(total)) != hipSuccess) return e;
+      if ((e = hipMemcpyAsync(d_path_offsets, h_path_offsets, 8ull * Qcap, hipMemcpyHostToDevice,
+                              s)) != hipSuccess)
+        return e;
+      a.pass = 2;
+      a.offsets = d_path_offsets;
+      a.arena_vertices = d_arena_v;
+      a.arena_labels = d_arena_l;
+      a.arena_edges = d_arena_e;
+      if ((e = paths_launch(a, ndone, s)) != hipSuccess) return e;
+      if ((e = hipMemcpyAsync(h_arena_v, d_arena_v, 4 * total, hipMemcpyDeviceToHost, s)) != hipSuccess)
+        return e;
+      if ((e = hipMemcpyAsync(h_arena_e, d_arena_e, 4 * total, hipMemcpyDeviceToHost, s)) != hipSuccess)
+        return e;
+      if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    }
   }
   return hipSuccess;
 }
@@ -380,14 +447,22 @@ void bfn_server_s::run() {
       auto it = std::find_if(active.begin(), active.end(),
                              [&](const Active& a) { return a.query == q; });
       if (it == active.end()) continue;
-      bfn_response r{};
+      Delivered d;
+      bfn_response& r = d.response;
+      r = bfn_response{};
       r.request_id = it->request_id;
       std::memcpy(&r.distance, &h_result[q], 4);
       r.reached = h_result[q] != kInfBits;
       r.path_length = options.want_paths ? h_summary[7ull * q + 3] : 0;
       r.submit_ns = it->submit_ns;
       r.complete_ns = t;
-      outbox.push_back(r);
+      if (options.want_paths && h_summary[7ull * q + 5] == 0u && h_summary[7ull * q + 6] == 1u) {
+        const std::uint64_t off = h_path_offsets[q];
+        const std::uint32_t len = h_summary[7ull * q + 3];
+        d.vertices.assign(h_arena_v + off, h_arena_v + off + len + 1);
+        d.edges.assign(h_arena_e + off, h_arena_e + off + len);
+      }
+      outbox.push_back(std::move(d));
       give_range(it->slot_begin, it->slot_count);
       free_queries.push_back(q);
       active.erase(it);
@@ -426,6 +501,7 @@ int32_t bfn_server_create(bfn_device device, bfn_graph graph, const bfn_server_o
     if (sv->h_done) (void)hipHostFree(sv->h_done);
     if (sv->h_result) (void)hipHostFree(sv->h_result);
     if (sv->h_summary) (void)hipHostFree(sv->h_summary);
+    if (sv->h_path_offsets) (void)hipHostFree(sv->h_path_offsets);
     if (sv->stream) (void)hipStreamDestroy(sv->stream);
     delete sv;
     return fail(code, m);
@@ -438,8 +514,8 @@ int32_t bfn_server_create(bfn_device device, bfn_graph graph, const bfn_server_o
   sv->slab_words = 2 * sv->max_v;
   // Push engines push boundary edges with direct atomics here (no proposal
   // area), which keeps more workgroups resident for small requests.
-  sv->work_words = std::max(o->engine == kJacobiPull ? sv->gview.max_stage_floats : 0u, sv->max_v);
-  sv->lds_bytes = 4ull * (2ull * sv->max_v + sv->work_words + kLdsTableWords + kLdsFlagWords);
+  sv->work_words = relax_work_words(sv->gview, o->engine == kJacobiPull, o->engine != kJacobiPull, false);
+  sv->lds_bytes = 4ull * relax_lds_words(sv->max_v, sv->work_words, sv->gview.max_neighbor_tiles);
   if (sv->lds_bytes > device_lds_limit(device))
     return bail(BFN_E_NOT_REPRESENTABLE, "tile neighbourhood exceeds the LDS budget");
   sv->block = o->block_size;
@@ -477,7 +553,8 @@ int32_t bfn_server_create(bfn_device device, bfn_graph graph, const bfn_server_o
             alloc(&sv->d_active, 4 * Q) && alloc(&sv->d_done, 4 * (Q + 1)) &&
             alloc(&sv->d_result, 4 * Q) && alloc(&sv->d_clean, 4 * Q) &&
             alloc(&sv->d_summary, 28 * Q) && alloc(&sv->d_lengths, 8 * Q) &&
-            alloc(&sv->d_frame_base, 8 * Q) && alloc(&sv->d_control, sizeof(DevControl)) &&
+            alloc(&sv->d_frame_base, 8 * Q) && alloc(&sv->d_path_offsets, 8 * Q) &&
+            alloc(&sv->d_control, sizeof(DevControl)) &&
             alloc(&sv->d_records, sizeof(AdmitRecord) * Q) &&
             alloc(&sv->d_upload_tiles, 4ull * sv->upload_tile_capacity);
   if (!ok) return bail(BFN_E_OUT_OF_MEMORY, "server allocation failed");
@@ -485,7 +562,8 @@ int32_t bfn_server_create(bfn_device device, bfn_graph graph, const bfn_server_o
       hipHostMalloc(reinterpret_cast<void**>(&sv->h_tiles), 4ull * sv->upload_tile_capacity) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&sv->h_done), 4 * (Q + 1)) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&sv->h_result), 4 * Q) != hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void**>(&sv->h_summary), 28 * Q) != hipSuccess)
+      hipHostMalloc(reinterpret_cast<void**>(&sv->h_summary), 28 * Q) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&sv->h_path_offsets), 8 * Q) != hipSuccess)
     return bail(BFN_E_OUT_OF_MEMORY, "pinned staging allocation failed");
   DevBatch& b = sv->view;
   b.query_count = sv->Qcap;
@@ -570,7 +648,40 @@ int32_t bfn_server_poll(bfn_server sv, bfn_response* responses, uint32_t capacit
                          [&] { return !sv->outbox.empty() || sv->stopping; });
   std::uint32_t n = 0;
   while (n < capacity && !sv->outbox.empty()) {
-    responses[n++] = sv->outbox.front();
+    responses[n++] = sv->outbox.front().response;
+    sv->outbox.pop_front();
+  }
+  *count = n;
+  if (!sv->fatal.empty()) return fail(BFN_E_HIP, sv->fatal);
+  return BFN_OK;
+}
+
+int32_t bfn_server_poll_paths(bfn_server sv, bfn_response* responses, uint32_t capacity,
+                              uint32_t* count, uint32_t timeout_ms, uint32_t* vertices,
+                              uint32_t* edge_ids, uint64_t vertex_capacity, uint64_t* path_offsets) {
+  if (!sv || !count || !path_offsets || (capacity && (!responses || !vertices || !edge_ids)))
+    return fail(BFN_E_INVALID_ARGUMENT, "null argument");
+  std::unique_lock<std::mutex> lock(sv->mutex);
+  sv->delivered.wait_for(lock, std::chrono::milliseconds(timeout_ms),
+                         [&] { return !sv->outbox.empty() || sv->stopping; });
+  std::uint32_t n = 0;
+  std::uint64_t used = 0;
+  path_offsets[0] = 0;
+  while (n < capacity && !sv->outbox.empty()) {
+    const Delivered& d = sv->outbox.front();
+    if (used + d.vertices.size() > vertex_capacity) {
+      if (n == 0) {
+        *count = 0;
+        return fail(BFN_E_INVALID_ARGUMENT, "vertex capacity below the next path's " +
+                                                std::to_string(d.vertices.size()) + " vertices");
+      }
+      break;
+    }
+    responses[n] = d.response;
+    std::copy(d.vertices.begin(), d.vertices.end(), vertices + used);
+    std::copy(d.edges.begin(), d.edges.end(), edge_ids + used);
+    used += d.vertices.size();
+    path_offsets[++n] = used;
     sv->outbox.pop_front();
   }
   *count = n;
@@ -602,6 +713,8 @@ int32_t bfn_server_destroy(bfn_server sv) {
   (void)hipHostFree(sv->h_done);
   (void)hipHostFree(sv->h_result);
   (void)hipHostFree(sv->h_summary);
+  (void)hipHostFree(sv->h_path_offsets);
+  sv->free_arena();
   (void)hipStreamDestroy(sv->stream);
   delete sv;
   return BFN_OK;

@@ -360,6 +360,17 @@ int32_t bfn_server_submit(bfn_server server, const bfn_request *request);
 /* Blocks for up to timeout_ms for responses; returns the count in *count. */
 int32_t bfn_server_poll(bfn_server server, bfn_response *responses, uint32_t capacity,
                         uint32_t *count, uint32_t timeout_ms);
+/* bfn_server_poll with each response's compact path (servers created with
+ * want_paths): response i's path vertices, source first, are
+ * vertices[path_offsets[i] .. path_offsets[i+1]) and its path_length edge ids
+ * start at edge_ids[path_offsets[i]] (an unreached target has an empty
+ * path).  path_offsets has capacity + 1 entries; responses are returned while
+ * their paths fit in vertex_capacity (BFN_E_INVALID_ARGUMENT when the next
+ * path alone does not). */
+int32_t bfn_server_poll_paths(bfn_server server, bfn_response *responses, uint32_t capacity,
+                              uint32_t *count, uint32_t timeout_ms, uint32_t *vertices,
+                              uint32_t *edge_ids, uint64_t vertex_capacity,
+                              uint64_t *path_offsets);
 int32_t bfn_server_stats(bfn_server server, uint64_t *launches, uint64_t *admitted,
                          uint64_t *completed);
 int32_t bfn_server_destroy(bfn_server server);

@@ -107,6 +107,9 @@ inline unsigned long long __ballot(int p) {
 inline std::uint32_t __shfl_xor(std::uint32_t v, int off, int = 64) {
   return static_cast<std::uint32_t>(emu_wave_exchange_read(v, (threadIdx.x % 64) ^ off));
 }
+inline std::uint32_t __shfl(std::uint32_t v, int src, int = 64) {
+  return static_cast<std::uint32_t>(emu_wave_exchange_read(v, static_cast<std::uint32_t>(src) % 64));
+}
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 inline float __uint_as_float(std::uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
 inline std::uint32_t __float_as_uint(float f) { std::uint32_t u; std::memcpy(&u, &f, 4); return u; }
