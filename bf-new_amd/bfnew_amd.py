@@ -208,6 +208,9 @@ HIP_SIGNATURES = {
                          C.c_uint64, C.POINTER(C.c_uint64)], C.c_int32),
     "bfn_batch_occupancy": ([C.c_void_p, C.c_void_p], C.c_int32),
     "bfn_batch_counters": ([C.c_void_p, C.POINTER(EngineCounters)], C.c_int32),
+    "bfn_batch_run_static": ([C.c_void_p, C.POINTER(BatchDesc), C.POINTER(RunOptions),
+                              C.POINTER(Tuning), C.c_void_p, C.c_void_p], C.c_int32),
+    "bfn_static_scratch_bytes": ([], C.c_uint64),
     "bfn_batch_timing": ([C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
                           C.POINTER(C.c_uint64)], C.c_int32),
     "bfn_server_create": ([C.c_void_p, C.c_void_p, C.POINTER(ServerOptions),

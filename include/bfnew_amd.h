@@ -323,6 +323,25 @@ typedef struct {
 } bfn_engine_counters;
 int32_t bfn_batch_counters(bfn_workspace workspace, bfn_engine_counters *out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the batch of bfn_batch_submit run to completion on the
+ * graph's stream with no allocation at all.  Every piece of per-batch device
+ * state lives in one static device array per GPU of
+ * bfn_static_scratch_bytes() bytes (64 MiB), held until the process ends;
+ * a batch that needs more returns BFN_E_OUT_OF_MEMORY before touching it.
+ * The batch is copied in from the caller's host arrays; per_query
+ * ([query_count]) and target_distances ([target_offsets[query_count]],
+ * +inf when unreached) are copied out before the call returns.  Tuning as
+ * bfn_batch_submit, except that use_graph and count_relaxations are refused
+ * and push engines always push boundary edges with direct atomics.  Every
+ * call on one GPU shares that GPU's array and keeps the last call's
+ * contents; calls that overlap on one GPU race on it. */
+int32_t bfn_batch_run_static(bfn_graph graph, const bfn_batch_desc *batch,
+                             const bfn_run_options *options, const bfn_tuning *tuning,
+                             bfn_query_status *per_query, float *target_distances);
+uint64_t bfn_static_scratch_bytes(void);
+
 /* Serving loop (continuous batching, north of the reference's single-batch
  * engines): a server owns a thread and a stream over a resident graph.
  * Requests are admitted by free workspace memory and join the running

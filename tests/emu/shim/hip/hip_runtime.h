@@ -22,6 +22,8 @@
 
 #define __global__
 #define __device__
+#define __align__(n) alignas(n)
+#define HIP_SYMBOL(x) (&(x))
 #define __host__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
@@ -181,6 +183,11 @@ hipError_t hipGetDeviceProperties(hipDeviceProp_t*, int);
 hipError_t hipDeviceSynchronize();
 hipError_t hipMalloc(void**, std::size_t);
 template <typename T> inline hipError_t hipMalloc(T** p, std::size_t n) { return hipMalloc(reinterpret_cast<void**>(p), n); }
+// Host memory stands in for device memory: a symbol's address is its own.
+inline hipError_t hipGetSymbolAddress(void** p, const void* symbol) {
+  *p = const_cast<void*>(symbol);
+  return hipSuccess;
+}
 hipError_t hipFree(void*);
 hipError_t hipHostMalloc(void**, std::size_t, unsigned = 0);
 template <typename T> inline hipError_t hipHostMalloc(T** p, std::size_t n, unsigned f = 0) { return hipHostMalloc(reinterpret_cast<void**>(p), n, f); }
