@@ -67,7 +67,7 @@ def profiled_traffic(workload, tune):
     if not entry:
         return None, None
     cfg = entry["config"]
-    keys = ("engine", "block", "sweeps", "boundary_mode", "use_graph", "min_waves")
+    keys = ("engine", "block", "sweeps", "boundary_mode", "use_graph", "min_waves", "claim_order")
     if any(float(cfg.get(k, 0)) != float(tune.get(k, 0)) for k in keys):
         return None, None
     return entry["hbm_bytes_per_step"], entry["source"]
@@ -89,7 +89,7 @@ def make_batch(hg, cfg):
 def tuned_defaults(tune_entry):
     t = dict(engine=bfn.JACOBI_PULL, block=256, sweeps=0, control=bfn.PERSISTENT,
              classes=None, items_per_claim=1, use_graph=0, graph_rounds=0, boundary_mode=0,
-             min_waves=0)
+             min_waves=0, claim_order=0)
     if tune_entry:
         t.update(tune_entry)
     return t
@@ -306,7 +306,7 @@ def negotiated(dev, tune):
         ws.submit(dg, batch, opts, bfn.tuning(tile_sweeps=tune["sweeps"], count_relaxations=1,
                                               items_per_claim=tune["items_per_claim"],
                                               boundary_mode=tune["boundary_mode"],
-                                              min_waves=tune["min_waves"]))
+                                              min_waves=tune["min_waves"], claim_order=tune.get("claim_order", 0)))
         st, per, _ = ws.status()
         relax_ms = ws.timing()[0]
         relaxations = sum(p.relaxations for p in per)
@@ -327,16 +327,24 @@ def negotiated(dev, tune):
 
 
 def copy_calibration():
-    """Three copies of 1 GiB between two device buffers: a kernel of known
-    traffic (1 GiB read, 1 GiB written per dispatch) against which the HBM
-    FETCH_SIZE / WRITE_SIZE counters are scaled."""
+    """Kernels of known traffic against which the HBM FETCH_SIZE / WRITE_SIZE
+    counters are scaled: three copies of 1 GiB between two device buffers
+    (1 GiB read and 1 GiB written per dispatch), then three sums of 1 GiB
+    (read only) and three fills of 1 GiB (write only), so that each counter's
+    scale is checked on a kernel that moves traffic in one direction."""
     import torch
     a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
     for _ in range(3):
         b.copy_(a)
     torch.cuda.synchronize()
-    log("copy calibration: 3 x 1073741824 bytes copied")
+    s = [a.sum() for _ in range(3)]
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.fill_(2.0)
+    torch.cuda.synchronize()
+    del s
+    log("copy calibration: 3 x 1073741824 bytes copied, summed and filled")
 
 
 EXPANSION_DRIVER = os.path.join(ROOT, "bf-new_amd", "lib", "bfnew_expansion_bench")
@@ -382,10 +390,11 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--engine", type=int, default=None, help="override the tuned engine")
+    ap.add_argument("--claim-order", type=int, default=None, help="override the tuned claim order")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported "
-              "(queries shard across GPUs as independent batches, see DESIGN.md).",
+              "(queries shard across GPUs as independent batches).",
               file=sys.stderr)
         sys.exit(2)
 
@@ -409,6 +418,8 @@ def main():
     if args.engine is not None:
         tune.update(engine=args.engine, classes=None)
         batch.engines = None
+    if args.claim_order is not None:
+        tune.update(claim_order=args.claim_order)
     opts = bfn.run_options(engine=tune["engine"], control_mode=tune["control"],
                            block_size=tune["block"])
     ws = bfn.Workspace(dev)
@@ -418,7 +429,7 @@ def main():
               bfn.tuning(tile_sweeps=tune["sweeps"], count_relaxations=1,
                          items_per_claim=tune["items_per_claim"], use_graph=tune["use_graph"],
                          graph_rounds=tune["graph_rounds"],
-                         boundary_mode=tune["boundary_mode"], min_waves=tune["min_waves"]))
+                         boundary_mode=tune["boundary_mode"], min_waves=tune["min_waves"], claim_order=tune.get("claim_order", 0)))
     st, per, _ = ws.status()
     if not st.converged:
         raise SystemExit("batch did not converge")
@@ -430,7 +441,7 @@ def main():
                                           use_graph=tune["use_graph"],
                                           graph_rounds=tune["graph_rounds"],
                                           boundary_mode=tune["boundary_mode"],
-                                          min_waves=tune["min_waves"]))
+                                          min_waves=tune["min_waves"], claim_order=tune.get("claim_order", 0)))
     elapsed, relax_ms, launches, paths_ms = timed_steps(ws, args.steps, args.warmup, paths=True)
     # one round kernel + one advance kernel per host-polled or replayed round
     round_launches = 1 if tune["control"] == bfn.PERSISTENT and not tune["use_graph"] else launches // 2
@@ -471,7 +482,7 @@ def main():
                    "unbounded_fraction": C3["unbounded"], "engine": bfn.ENGINE_NAMES[tune["engine"]],
                    "per_class_engines": tune.get("classes"), "block": tune["block"],
                    "tile_sweeps": tune["sweeps"], "boundary_mode": tune["boundary_mode"],
-                   "min_waves": tune["min_waves"],
+                   "min_waves": tune["min_waves"], "claim_order": tune["claim_order"],
                    "control": bfn.CONTROL_NAMES[tune["control"]] +
                    (f"+hipgraph({tune['graph_rounds']} rounds)" if tune["use_graph"] else ""),
                    "relaxations_per_step": relax_per_step,

@@ -73,7 +73,8 @@ def time_candidate(dev, dg, batch, cand, reps):
                           use_graph=cand.get("use_graph", 0),
                           graph_rounds=cand.get("graph_rounds", 0),
                           boundary_mode=cand.get("boundary_mode", 0),
-                          min_waves=cand.get("min_waves", 0))
+                          min_waves=cand.get("min_waves", 0),
+                          claim_order=cand.get("claim_order", 0))
         ws.submit(dg, batch, opts, tune)
         st, _, _ = ws.status()
         if not st.converged:
@@ -262,7 +263,8 @@ def main():
         log(f"{bfn.ENGINE_NAMES[cand['engine']]:>20} block {cand['block']:4d} sweeps "
             f"{cand['sweeps']} boundary {cand['boundary_mode']} waves {cand['min_waves']} "
             f"ipc {cand['items_per_claim']} control "
-            f"{bfn.CONTROL_NAMES[cand['control']]}{'+graph' if cand.get('use_graph') else ''}: "
+            f"{bfn.CONTROL_NAMES[cand['control']]}{'+graph' if cand.get('use_graph') else ''}"
+            f"{' slot-order' if cand.get('claim_order') else ''}: "
             f"{ms:.3f} ms, {rounds} rounds")
         return rounds
 
@@ -272,7 +274,10 @@ def main():
             # Push schedules differ by a round or two between runs: capture a
             # small margin so that replays rarely need the host-polled tail.
             margin = 0 if cand["engine"] == bfn.JACOBI_PULL else 2
-            trial(dict(cand, control=bfn.CHUNKED, use_graph=1, graph_rounds=rounds + margin))
+            twin = dict(cand, control=bfn.CHUNKED, use_graph=1, graph_rounds=rounds + margin)
+            trial(twin)
+            # and in slot order (queue bits compacted between round launches)
+            trial(dict(twin, claim_order=1))
 
     for cand in candidates(engines, blocks, args.quick):
         with_graph_twin(cand)
@@ -300,6 +305,7 @@ def main():
             {bfn.ENGINE_NAMES[e]: round(v, 3) for e, v in timing.items()})
     entry = {k: best[k] for k in ("engine", "block", "sweeps", "control", "items_per_claim",
                                   "boundary_mode", "min_waves")}
+    entry["claim_order"] = best.get("claim_order", 0)
     entry["use_graph"] = best.get("use_graph", 0)
     entry["graph_rounds"] = best.get("graph_rounds", 0)
     entry["ms"] = best["ms"]
@@ -327,7 +333,7 @@ def main():
         json.dump(table, f, indent=1, sort_keys=True)
     with open(os.path.join(args.out, f"trials_{args.workload}.tsv"), "w") as f:
         cols = ["scope", "engine", "block", "sweeps", "boundary_mode", "min_waves", "items_per_claim",
-                "control", "use_graph", "graph_rounds", "rounds", "ms"]
+                "control", "use_graph", "graph_rounds", "claim_order", "rounds", "ms"]
         f.write("\t".join(cols) + "\n")
         for t in trials:
             f.write("\t".join(str(t.get(c, 0)) for c in cols) + "\n")

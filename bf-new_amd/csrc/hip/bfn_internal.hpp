@@ -133,6 +133,12 @@ struct DevControl {
   unsigned long long algorithmic_bytes;  // modelled HBM bytes of the visits
   unsigned long long closure_sweeps;     // tile-closure sweeps (one per block pass)
   unsigned long long phase_cycles[kPhaseCount];  // summed over workgroups
+  // Slot-order claims: the compacted list of the round is cut into one
+  // segment per XCD group, [seg_begin[x], seg_begin[x+1]) (read once per
+  // round launch); seg_claim[x * kBarrierStride] is the next unclaimed
+  // position of segment x, each counter on its own 128-byte line.
+  std::uint32_t seg_begin[kBarrierGroups + 1];
+  alignas(128) std::uint32_t seg_claim[kBarrierGroups * kBarrierStride];
   // Arrival counters of the two-level grid barrier, one per XCD group of
   // workgroups (blockIdx % 8), each on its own 128-byte line.
   alignas(128) std::uint32_t group_arrivals[kBarrierGroups * kBarrierStride];
@@ -162,6 +168,7 @@ struct DevBatch {
   std::uint32_t instrumentation;
   std::uint32_t boundary_mode;    // push: 0 LDS proposals when they fit, 1 direct atomics
   std::uint32_t min_waves;        // kernel variant bounded to >= this many waves per SIMD
+  std::uint32_t ordered;          // slot-order claims: queue as bits in `active`
   const std::uint32_t* tile_begin;      // [Q]   first position of q in query_tiles
   const std::uint32_t* tile_end;        // [Q]   one past q's last position
   const std::uint32_t* query_tiles;     // [.]   q's selected tiles, sorted
@@ -183,6 +190,10 @@ struct DevBatch {
   // null when visits look the offsets up through slot_dist.
   unsigned long long* slot_out_off;
   std::uint32_t* stamp;                 // [S]
+  // Slot-order claims: bit s of `active` queues slot s for the next round
+  // (ceil(S/32) words, sharing the stamp buffer, which that mode leaves
+  // unused); the compacted list of a round is lists[0 .. n).
+  std::uint32_t* active;
   std::uint32_t* lists;                 // [3*S]
   std::uint32_t* dist;                  // float bits pool
   std::uint32_t* last_change;           // [Q] 1 + last round with a decrease
@@ -246,6 +257,9 @@ __device__ __forceinline__ std::uint32_t atomic_max_agent(std::uint32_t* p, std:
 __device__ __forceinline__ std::uint32_t atomic_add_agent(std::uint32_t* p, std::uint32_t v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ void atomic_or_agent(std::uint32_t* p, std::uint32_t v) {
+  __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 __device__ __forceinline__ float as_float(std::uint32_t b) { return __uint_as_float(b); }
 __device__ __forceinline__ std::uint32_t as_bits(float f) { return __float_as_uint(f); }
@@ -254,6 +268,10 @@ __device__ __forceinline__ std::uint32_t as_bits(float f) { return __float_as_ui
 // position, and the query's pending work (when tracked) counts the item.
 __device__ __forceinline__ void enqueue(const DevBatch& b, std::uint32_t slot,
                                         std::uint32_t round) {
+  if (b.ordered) {
+    atomic_or_agent(&b.active[slot >> 5], 1u << (slot & 31u));
+    return;
+  }
   const std::uint32_t tag = round + 1;
   if (atomic_max_agent(&b.stamp[slot], tag) < tag) {
     const std::uint32_t li = round % 3;
@@ -270,6 +288,12 @@ __device__ __forceinline__ void enqueue(const DevBatch& b, std::uint32_t slot,
 // one per slot).
 __device__ __forceinline__ void enqueue_wave(const DevBatch& b, std::uint32_t slot, bool want,
                                              std::uint32_t round) {
+  if (b.ordered) {
+    // A bit per slot: setting it twice queues the slot once, and no lane
+    // waits for a returned value.
+    if (want) atomic_or_agent(&b.active[slot >> 5], 1u << (slot & 31u));
+    return;
+  }
   const std::uint32_t tag = round + 1;
   const std::uint32_t lane = threadIdx.x & 63u;
   const bool first = want && atomic_max_agent(&b.stamp[slot], tag) < tag;

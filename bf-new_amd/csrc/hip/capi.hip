@@ -811,6 +811,7 @@ void bind_view(bfn_workspace ws, const BatchPlanHost& p, const std::size_t* seg,
   b.instrumentation = o->instrumentation;
   b.boundary_mode = t->boundary_mode;
   b.min_waves = t->min_waves;
+  b.ordered = t->claim_order;
   b.tile_begin = reinterpret_cast<const std::uint32_t*>(m + seg[0]);
   b.tile_end = b.tile_begin + 1;
   b.query_tiles = reinterpret_cast<const std::uint32_t*>(m + seg[1]);
@@ -830,6 +831,7 @@ void bind_view(bfn_workspace ws, const BatchPlanHost& p, const std::size_t* seg,
   b.slot_out_off = ws->slot_out_off.as<unsigned long long>();
   b.slot_desc = ws->slot_desc.as<SlotDesc>();
   b.stamp = ws->stamp.as<std::uint32_t>();
+  b.active = b.stamp;
   b.lists = ws->lists.as<std::uint32_t>();
   b.dist = ws->pool.as<std::uint32_t>();
   b.last_change = ws->last_change.as<std::uint32_t>();
@@ -985,9 +987,14 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
   if (int32_t rc = validate_options(o)) return rc;
   bfn_tuning t;
   if (tuning) t = *tuning; else bfn_tuning_default(&t);
-  if (t.tile_sweeps > 1 || t.boundary_mode > 1 || t.use_graph > 1 || t.min_waves > 8)
+  if (t.tile_sweeps > 1 || t.boundary_mode > 1 || t.use_graph > 1 || t.min_waves > 8 ||
+      t.claim_order > 1)
     return fail(BFN_E_INVALID_ARGUMENT,
-                "invalid tuning (tile_sweeps, boundary_mode, use_graph, min_waves)");
+                "invalid tuning (tile_sweeps, boundary_mode, use_graph, min_waves, claim_order)");
+  if (t.claim_order && o->control_mode == 0)
+    return fail(BFN_E_INVALID_ARGUMENT,
+                "claim_order 1 compacts the queue between round launches: it needs per-round "
+                "control (control_mode 1 or 2)");
   BatchPlanHost p;
   if (int32_t rc = plan_batch(graph, d, o, p)) return rc;
   BFN_HIP(hipSetDevice(ws->device->ordinal));

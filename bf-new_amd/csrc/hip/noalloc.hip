@@ -155,8 +155,11 @@ int32_t bfn_batch_run_static(bfn_graph graph, const bfn_batch_desc* d, const bfn
   if (int32_t rc = check_run_options(o)) return rc;
   bfn_tuning t;
   if (tuning) t = *tuning; else bfn_tuning_default(&t);
-  if (t.tile_sweeps > 1 || t.boundary_mode > 1 || t.min_waves > 8)
-    return fail(BFN_E_INVALID_ARGUMENT, "invalid tuning (tile_sweeps, boundary_mode, min_waves)");
+  if (t.tile_sweeps > 1 || t.boundary_mode > 1 || t.min_waves > 8 || t.claim_order > 1)
+    return fail(BFN_E_INVALID_ARGUMENT,
+                "invalid tuning (tile_sweeps, boundary_mode, min_waves, claim_order)");
+  if (t.claim_order && o->control_mode == 0)
+    return fail(BFN_E_INVALID_ARGUMENT, "claim_order 1 needs per-round control (control_mode 1 or 2)");
   if (t.use_graph || t.count_relaxations)
     return fail(BFN_E_INVALID_ARGUMENT,
                 "graph replay and relaxation counts are not available without allocation");
@@ -245,6 +248,7 @@ int32_t bfn_batch_run_static(bfn_graph graph, const bfn_batch_desc* d, const bfn
   b.instrumentation = o->instrumentation;
   b.boundary_mode = 1;
   b.min_waves = t.min_waves;
+  b.ordered = t.claim_order;
   b.tile_begin = u32(o_tile_off);
   b.tile_end = u32(o_tile_off) + 1;
   b.query_tiles = u32(o_tiles);
@@ -264,6 +268,7 @@ int32_t bfn_batch_run_static(bfn_graph graph, const bfn_batch_desc* d, const bfn
   b.slot_out_off = nullptr;
   b.slot_desc = reinterpret_cast<const SlotDesc*>(m + o_desc);
   b.stamp = u32(o_stamp);
+  b.active = u32(o_stamp);
   b.lists = u32(o_lists);
   b.dist = u32(o_pool);
   b.last_change = u32(o_last);

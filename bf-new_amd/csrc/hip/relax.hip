@@ -275,6 +275,17 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
   const std::uint32_t out_base = d.nbr_base;
   std::uint32_t* column = b.dist + ((static_cast<unsigned long long>(d.dist_hi) << 32) | d.dist_lo);
   std::uint32_t* snapshot = column + npt;
+  // CSR block of T: internal records [e0, ei), boundary records [ei, e1).
+  const std::uint32_t e0 = d.e0;
+  const std::uint32_t e1 = d.e1;
+  const std::uint32_t ei = d.ei;
+  const bool resident = ei - e0 <= static_cast<std::uint32_t>(B * EPT);
+  // The internal records stay in registers from a previous visit of T;
+  // otherwise they are requested first, so that they travel together with
+  // the label loads below (one round trip for both).
+  const std::uint32_t key = 2 * T + 1;
+  if (resident && regs_key != key) load_records<B, EPT>(internal, g.csr_rec, e0, ei);
+  regs_key = resident ? key : kNone;
 
   const bool proposals = b.boundary_mode == 0;
   for (std::uint32_t k = tid; k < KO; k += B) {
@@ -313,11 +324,6 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
   __syncthreads();
   st.mark(0);
 
-  // CSR block of T: internal records [e0, ei), boundary records [ei, e1).
-  const std::uint32_t e0 = d.e0;
-  const std::uint32_t e1 = d.e1;
-  const std::uint32_t ei = d.ei;
-  const bool resident = ei - e0 <= static_cast<std::uint32_t>(B * EPT);
   if (tid == 0) {
     // internal records once, the column (and the snapshot for frontier)
     // once; proposals stream every boundary record, direct mode reads the
@@ -326,10 +332,6 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
                 (proposals ? 8ull * (e1 - ei) : 4ull * npt);
     st.edges += (ei - e0) + (proposals ? e1 - ei : 0u);
   }
-  // The internal records stay in registers from a previous visit of T.
-  const std::uint32_t key = 2 * T + 1;
-  if (resident && regs_key != key) load_records<B, EPT>(internal, g.csr_rec, e0, ei);
-  regs_key = resident ? key : kNone;
   st.mark(1);
   // Internal edges to the tile's local fixed point, from frontier sources.
   for (;;) {
@@ -407,7 +409,9 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
     // (cacheable, possibly stale) loads: labels only fall, so a stale label
     // is never below the true one and the filter never drops a decrease; a
     // neighbour is marked when its read label was above a candidate, which
-    // covers every actual decrease.
+    // covers every actual decrease.  (A returning atomic per candidate in
+    // place of the read measured 1.6x slower on 128x128x512 tiles: device-
+    // scope atomics are performed past the L2, at the memory side.)
     for (std::uint32_t base = 0; base < npt; base += B) {
       const std::uint32_t i = base + tid;
       const bool push = i < npt && l.own[i] < l.work[i];
@@ -503,30 +507,81 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
 template <int B, int EPT, bool I>
 __device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch& b, std::uint32_t r,
                               std::uint32_t n, const Lds& l) {
-  const std::uint32_t li = r % 3;
   const std::uint32_t chunk = b.items_per_claim ? b.items_per_claim : 1u;
   RecordRegs<B, EPT> regs;
   std::uint32_t regs_key = kNone;  // tile and orientation of the records in regs
   VisitStats<I> st;
-  for (;;) {
-    if (threadIdx.x == 0) l.flags[0] = atomic_add_agent(&b.control->claim[li], chunk);
+  auto visit = [&](std::uint32_t s) {
+    const SlotDesc d = load_slot_desc(g, b, s);
+    if (d.engine == kJacobiPull) {
+      visit_pull<B, EPT, I>(g, b, d, r, l, regs, regs_key, st);
+    } else {
+      visit_push<B, EPT, I>(g, b, d, r, d.engine == kFrontierPush, l, regs, regs_key, st);
+    }
+    ++st.items;
     __syncthreads();
-    const std::uint32_t start = l.flags[0];
-    __syncthreads();
-    if (start >= n) break;
-    const std::uint32_t end = min(start + chunk, n);
-    for (std::uint32_t i = start; i < end; ++i) {
-      const std::uint32_t s = b.lists[static_cast<std::size_t>(li) * b.slot_count + i];
-      const SlotDesc d = load_slot_desc(g, b, s);
-      if (d.engine == kJacobiPull) {
-        visit_pull<B, EPT, I>(g, b, d, r, l, regs, regs_key, st);
-      } else {
-        visit_push<B, EPT, I>(g, b, d, r, d.engine == kFrontierPush, l, regs, regs_key, st);
+    // the visit's own enqueues are issued before its item is retired
+    if (b.pending && threadIdx.x == 0) atomic_add_agent(&b.pending[d.query], 0xFFFFFFFFu);
+  };
+  if (b.ordered) {
+    // Slot order: the round's list is cut into one segment per XCD group;
+    // a workgroup claims from its own group's segment (blockIdx % 8: the
+    // dispatcher places workgroups on the XCDs round-robin), then from the
+    // others in turn.  Consecutive slots share a tile, so the slots of one
+    // tile are visited by one XCD, close together in time, and a chunk of
+    // them by one workgroup (internal records stay in registers).  Thread 0
+    // keeps the next claim in flight while a chunk is visited.
+    DevControl* c = b.control;
+    const std::uint32_t home = blockIdx.x % kBarrierGroups;
+    std::uint32_t tried = 0;  // thread 0: segments exhausted so far
+    std::uint32_t ahead = 0;  // thread 0: claim in flight on segment home + tried
+    // Segment ends, read once: the claim counters sit on lines of their own,
+    // which no plain load shares.
+    std::uint32_t seg_end[kBarrierGroups];
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (std::uint32_t x = 0; x < kBarrierGroups; ++x) seg_end[x] = c->seg_begin[x + 1];
+      ahead = atomic_add_agent(&c->seg_claim[home * kBarrierStride], chunk);
+    }
+    for (;;) {
+      if (threadIdx.x == 0) {
+        std::uint32_t start = kNone, end = 0;
+        while (tried < kBarrierGroups) {
+          const std::uint32_t y = (home + tried) % kBarrierGroups;
+          std::uint32_t e = seg_end[0];
+#pragma unroll
+          for (std::uint32_t x = 1; x < kBarrierGroups; ++x)
+            if (y == x) e = seg_end[x];
+          if (ahead < e) {
+            start = ahead;
+            end = min(ahead + chunk, e);
+            ahead = atomic_add_agent(&c->seg_claim[y * kBarrierStride], chunk);
+            break;
+          }
+          if (++tried < kBarrierGroups)
+            ahead = atomic_add_agent(&c->seg_claim[((home + tried) % kBarrierGroups) * kBarrierStride],
+                                     chunk);
+        }
+        l.flags[0] = start;
+        l.flags[2] = end;
       }
-      ++st.items;
       __syncthreads();
-      // the visit's own enqueues are issued before its item is retired
-      if (b.pending && threadIdx.x == 0) atomic_add_agent(&b.pending[d.query], 0xFFFFFFFFu);
+      const std::uint32_t start = l.flags[0], end = l.flags[2];
+      __syncthreads();
+      if (start == kNone) break;
+      for (std::uint32_t i = start; i < end; ++i) visit(b.lists[i]);
+    }
+  } else {
+    const std::uint32_t li = r % 3;
+    for (;;) {
+      if (threadIdx.x == 0) l.flags[0] = atomic_add_agent(&b.control->claim[li], chunk);
+      __syncthreads();
+      const std::uint32_t start = l.flags[0];
+      __syncthreads();
+      if (start >= n) break;
+      const std::uint32_t end = min(start + chunk, n);
+      for (std::uint32_t i = start; i < end; ++i)
+        visit(b.lists[static_cast<std::size_t>(li) * b.slot_count + i]);
     }
   }
   if constexpr (I) {
@@ -591,7 +646,7 @@ __global__ __launch_bounds__(B, MW) void relax_round_kernel(DevGraph g, DevBatch
   DevControl* c = b.control;
   if (load_agent(&c->done)) return;
   const std::uint32_t r = load_agent(&c->round);
-  const std::uint32_t n = load_agent(&c->list_count[r % 3]);
+  const std::uint32_t n = b.ordered ? 0u : load_agent(&c->list_count[r % 3]);
   process_round<B, EPT, I>(g, b, r, n, l);
 }
 
@@ -611,6 +666,103 @@ __global__ void relax_advance_kernel(DevBatch b) {
   } else if (b.maximum_rounds && r + 1 >= b.maximum_rounds) {
     c->done = 1;
     c->stop_reason = kStopMaximumRounds;
+  }
+}
+
+// Between rounds in slot order (one workgroup): count the round as
+// relax_advance_kernel does, then compact the queued bits into lists[0, n)
+// and clear them.  Slots are taken in groups of 64 (two bit words); group k
+// belongs to segment k % 8, and the list holds segment 0's groups in order,
+// then segment 1's, ..., so each segment is in slot order.  first = 1 (after
+// seeding) compacts only.
+constexpr std::uint32_t kCompactGroupsPerThread = 8;
+
+__global__ __launch_bounds__(1024) void ordered_advance_kernel(DevBatch b, std::uint32_t first) {
+  __shared__ std::uint32_t scan[1024];
+  __shared__ std::uint32_t seg_start[kBarrierGroups];
+  DevControl* c = b.control;
+  if (c->done) return;
+  const std::uint32_t tid = threadIdx.x;
+  const std::uint32_t W = (b.slot_count + 31u) / 32u;
+  const std::uint32_t NG = (W + 1u) / 2u;
+  // Groups of segment x: x, x + 8, ...; its first position in the virtual
+  // order (segment 0's groups first) is vstart[x].
+  std::uint32_t vstart[kBarrierGroups + 1];
+  vstart[0] = 0;
+#pragma unroll
+  for (std::uint32_t x = 0; x < kBarrierGroups; ++x)
+    vstart[x + 1] = vstart[x] + (NG > x ? (NG - 1u - x) / kBarrierGroups + 1u : 0u);
+  if (tid < kBarrierGroups) seg_start[tid] = kNone;
+  std::uint32_t carry = 0;
+  for (std::uint32_t base = 0; base < NG; base += 1024u * kCompactGroupsPerThread) {
+    const std::uint32_t v0 = base + tid * kCompactGroupsPerThread;
+    std::uint32_t w0[kCompactGroupsPerThread], w1[kCompactGroupsPerThread],
+        grp[kCompactGroupsPerThread], seg[kCompactGroupsPerThread];
+    std::uint32_t total = 0;
+#pragma unroll
+    for (std::uint32_t j = 0; j < kCompactGroupsPerThread; ++j) {
+      const std::uint32_t v = v0 + j;
+      w0[j] = w1[j] = 0;
+      grp[j] = kNone;
+      seg[j] = 0;
+      if (v < NG) {
+        std::uint32_t x = 0;
+        while (v >= vstart[x + 1]) ++x;
+        seg[j] = x;
+        grp[j] = x + kBarrierGroups * (v - vstart[x]);
+        w0[j] = b.active[2 * grp[j]];
+        if (2 * grp[j] + 1 < W) w1[j] = b.active[2 * grp[j] + 1];
+      }
+    }
+#pragma unroll
+    for (std::uint32_t j = 0; j < kCompactGroupsPerThread; ++j) total += __popc(w0[j]) + __popc(w1[j]);
+    scan[tid] = total;
+    __syncthreads();
+    for (std::uint32_t off = 1; off < 1024; off <<= 1) {
+      const std::uint32_t add = tid >= off ? scan[tid - off] : 0u;
+      __syncthreads();
+      scan[tid] += add;
+      __syncthreads();
+    }
+    std::uint32_t pos = carry + scan[tid] - total;
+    const std::uint32_t chunk_total = scan[1023];
+#pragma unroll
+    for (std::uint32_t j = 0; j < kCompactGroupsPerThread; ++j) {
+      if (grp[j] == kNone) continue;
+      if (v0 + j == vstart[seg[j]]) seg_start[seg[j]] = pos;
+      const std::uint32_t s0 = 64u * grp[j];
+      for (std::uint32_t m = w0[j]; m; m &= m - 1u) b.lists[pos++] = s0 + __ffs(m) - 1u;
+      for (std::uint32_t m = w1[j]; m; m &= m - 1u) b.lists[pos++] = s0 + 32u + __ffs(m) - 1u;
+      if (w0[j]) b.active[2 * grp[j]] = 0;
+      if (w1[j]) b.active[2 * grp[j] + 1] = 0;
+    }
+    carry += chunk_total;
+    __syncthreads();
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const std::uint32_t n = carry;
+    // Segments without groups (NG < 8) are empty and sit at the end.
+    for (std::uint32_t x = 0; x < kBarrierGroups; ++x) {
+      const std::uint32_t at = seg_start[x] == kNone ? n : seg_start[x];
+      c->seg_begin[x] = at;
+      c->seg_claim[x * kBarrierStride] = at;
+    }
+    c->seg_begin[kBarrierGroups] = n;
+    c->list_count[0] = n;
+    if (b.instrumentation && n > c->max_list) c->max_list = n;
+    if (!first) {
+      const std::uint32_t r = c->round;
+      c->round = r + 1;
+      c->rounds_executed = r + 1;
+      if (n == 0) {
+        c->done = 1;
+        c->stop_reason = kStopConverged;
+      } else if (b.maximum_rounds && r + 1 >= b.maximum_rounds) {
+        c->done = 1;
+        c->stop_reason = kStopMaximumRounds;
+      }
+    }
   }
 }
 
@@ -667,7 +819,7 @@ __global__ void prep_slots_kernel(DevGraph g, DevBatch b) {
     b.slot_out[obs + (k - o0)] = ns;
     if (b.slot_out_off) b.slot_out_off[obs + (k - o0)] = ns == kNone ? ~0ull : b.slot_dist[ns];
   }
-  b.stamp[s] = 0;
+  b.stamp[s] = 0;  // also every queue-bit word of slot-order claims (ceil(S/32) <= S)
   b.last_change[q] = 0;
   if (b.slot_desc) const_cast<SlotDesc*>(b.slot_desc)[s] = make_slot_desc(g, b, s);
 }
@@ -722,7 +874,10 @@ hipError_t launch_relax(bool persistent, bool cooperative, std::uint32_t grid,
                      g, b, max_v, work_words);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(relax_advance_kernel, dim3(1), dim3(1), 0, stream, b);
+  if (b.ordered)
+    hipLaunchKernelGGL(ordered_advance_kernel, dim3(1), dim3(1024), 0, stream, b, 0u);
+  else
+    hipLaunchKernelGGL(relax_advance_kernel, dim3(1), dim3(1), 0, stream, b);
   return hipGetLastError();
 }
 
@@ -792,6 +947,7 @@ hipError_t prep_launch(const DevGraph& g, const DevBatch& b, std::uint64_t pool_
   if (b.query_count)
     hipLaunchKernelGGL(seed_kernel, dim3((b.query_count + 127) / 128), dim3(128), 0, stream, g,
                        b);
+  if (b.ordered) hipLaunchKernelGGL(ordered_advance_kernel, dim3(1), dim3(1024), 0, stream, b, 1u);
   return hipGetLastError();
 }
 

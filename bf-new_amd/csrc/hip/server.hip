@@ -574,6 +574,8 @@ int32_t bfn_server_create(bfn_device device, bfn_graph graph, const bfn_server_o
   b.instrumentation = 0;
   b.boundary_mode = 1;
   b.min_waves = 0;
+  b.ordered = 0;  // arrival-order lists: admitted work joins the running round
+  b.active = nullptr;
   // A request's slots form one contiguous range of the arena, in tile
   // order, so positions and slots coincide.
   b.tile_begin = sv->d_begin;

@@ -134,11 +134,18 @@ typedef struct {
                                     per destination in LDS, then one coalesced
                                     compare and an atomic per improved label
                                     (when the LDS holds the proposals);
-                                    1 = one global atomic per boundary edge */
+                                    1 = per boundary edge, a read of the
+                                    neighbour's label, then a global atomic
+                                    when the candidate is below it */
   uint32_t min_waves;            /* 0 = compiler default; 6 or 8 = the 256- and
                                     512-thread kernels compiled for at least that
                                     many waves per SIMD */
-  uint32_t reserved;
+  uint32_t claim_order;          /* per-round launches only: 0 = work items in
+                                    arrival order (one list per round); 1 = work
+                                    items in slot order (tile-major), split into
+                                    eight segments that the workgroups of each
+                                    XCD claim first (queued as bits, compacted
+                                    between rounds) */
 } bfn_tuning;
 
 /* The reference DeviceGraphLayout32 (device_layout.hpp:72-93) plus the tile

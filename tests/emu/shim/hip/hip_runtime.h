@@ -34,6 +34,7 @@
 #define __hip_atomic_fetch_add emu__hip_atomic_fetch_add
 #define __hip_atomic_fetch_min emu__hip_atomic_fetch_min
 #define __hip_atomic_fetch_max emu__hip_atomic_fetch_max
+#define __hip_atomic_fetch_or emu__hip_atomic_fetch_or
 #define __HIP_MEMORY_SCOPE_WORKGROUP 0
 
 struct dim3 {
@@ -113,6 +114,8 @@ inline std::uint32_t __shfl(std::uint32_t v, int src, int = 64) {
   return static_cast<std::uint32_t>(emu_wave_exchange_read(v, static_cast<std::uint32_t>(src) % 64));
 }
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+inline int __popc(unsigned x) { return __builtin_popcount(x); }
+inline int __ffs(unsigned x) { return __builtin_ffs(static_cast<int>(x)); }
 inline float __uint_as_float(std::uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
 inline std::uint32_t __float_as_uint(float f) { std::uint32_t u; std::memcpy(&u, &f, 4); return u; }
 inline void __threadfence() {}
@@ -155,6 +158,7 @@ template <typename T> inline void emu__hip_atomic_store(T* p, T v, int, int) { e
 template <typename T> inline T emu__hip_atomic_fetch_add(T* p, T v, int, int) { return atomicAdd(p, v); }
 template <typename T> inline T emu__hip_atomic_fetch_min(T* p, T v, int, int) { return atomicMin(p, v); }
 template <typename T> inline T emu__hip_atomic_fetch_max(T* p, T v, int, int) { return atomicMax(p, v); }
+template <typename T> inline T emu__hip_atomic_fetch_or(T* p, T v, int, int) { return atomicOr(p, v); }
 
 // ---- runtime API ----------------------------------------------------------
 enum hipError_t { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorOutOfMemory = 2, hipErrorNotReady = 600 };

@@ -272,6 +272,44 @@ def test_hipgraph_replay(device):
         ws.close()
 
 
+def test_slot_order_claims(device):
+    """claim_order 1 (queue bits compacted between rounds into slot-order
+    segments claimed per XCD group): every engine, per-round and chunked
+    control with hipGraph replay, one and three items per claim, bit-exact
+    against the oracle on two runs; point Jacobi takes exactly the rounds of
+    arrival-order claims (its rounds do not depend on the order of visits);
+    a round limit stops it the same way; persistent control refuses it."""
+    hg, dg = synthetic(device, 32, 32, 128, 6)
+    view = hg.view()
+    qs, batch = c2_batch(hg, 12, seed=13)
+    refs = [oracle_query(view, [q.source], batch.tiles[i])[0] for i, q in enumerate(qs)]
+    for eng in ENGINES:
+        for ctl, graph in ((bfn.PER_ROUND, 0), (bfn.CHUNKED, 1)):
+            for ipc in (1, 3):
+                ws = run(device, dg, batch, eng, ctl, 0, items_per_claim=ipc, use_graph=graph,
+                         graph_rounds=4 if graph else 0, boundary_mode=1, claim_order=1)
+                for rep in range(2):
+                    assert ws.status()[0].converged == 1, (eng, ctl, ipc, rep)
+                    for i in range(len(qs)):
+                        assert np.array_equal(bits(ws.distances(i)), bits(refs[i])), (eng, ctl, ipc, rep, i)
+                    ws.rerun()
+                ws.close()
+    rounds = []
+    for order in (0, 1):
+        ws = run(device, dg, batch, bfn.JACOBI_PULL, bfn.PER_ROUND, 1, claim_order=order)
+        st, per, _ = ws.status()
+        rounds.append((st.rounds_completed, [p.rounds_completed for p in per]))
+        ws.close()
+    assert rounds[0] == rounds[1]
+    ws = run(device, dg, batch, bfn.JACOBI_PULL, bfn.CHUNKED, 1, max_rounds=2, claim_order=1)
+    st, _, _ = ws.status()
+    assert st.stop_reason == 2 and st.converged == 0 and st.rounds_completed == 2
+    ws.close()
+    with pytest.raises(bfn.BfnError) as refused:
+        run(device, dg, batch, bfn.FRONTIER_PUSH, bfn.PERSISTENT, 0, claim_order=1)
+    assert refused.value.code == 1
+
+
 def test_relaxation_counts_match_oracle(device):
     hg, dg = synthetic(device, 32, 32, 128, 8)
     view = hg.view()
@@ -304,6 +342,9 @@ def test_c3_batch_certificate(device):
                               items_per_claim=2, use_graph=1, graph_rounds=36, boundary_mode=1),
         "dense_persistent": run(device, dg, batch, bfn.DENSE_CHAOTIC_PUSH, bfn.PERSISTENT, 0,
                                 boundary_mode=1),
+        "frontier_slot_order": run(device, dg, batch, bfn.FRONTIER_PUSH, bfn.CHUNKED, 0, block=256,
+                                   items_per_claim=2, use_graph=1, graph_rounds=36, boundary_mode=1,
+                                   claim_order=1),
     }
     for name, ws in runs.items():
         assert ws.status()[0].converged == 1, name
@@ -315,7 +356,7 @@ def test_c3_batch_certificate(device):
         for name, ws in runs.items():
             assert np.array_equal(bits(ws.region_distances(i)), bits(ref)), (name, i)
     jac = runs["jacobi_persistent"].paths()
-    for name in ("frontier_tuned", "dense_persistent"):
+    for name in ("frontier_tuned", "dense_persistent", "frontier_slot_order"):
         assert [p["distance"] for p in runs[name].paths()] == [p["distance"] for p in jac], name
     for ws in runs.values():
         ws.close()
