@@ -67,7 +67,8 @@ def profiled_traffic(workload, tune):
     if not entry:
         return None, None
     cfg = entry["config"]
-    keys = ("engine", "block", "sweeps", "boundary_mode", "use_graph", "min_waves", "claim_order")
+    keys = ("engine", "block", "sweeps", "boundary_mode", "use_graph", "min_waves", "claim_order",
+            "push_window")
     if any(float(cfg.get(k, 0)) != float(tune.get(k, 0)) for k in keys):
         return None, None
     return entry["hbm_bytes_per_step"], entry["source"]
@@ -86,10 +87,15 @@ def make_batch(hg, cfg):
     return qs, bfn.Batch([[q.source] for q in qs], [[q.target] for q in qs], tiles)
 
 
+def scheduling(tune):
+    """The claim order and push window of a tuning entry (absent: off)."""
+    return dict(claim_order=tune.get("claim_order", 0), push_window=tune.get("push_window", 0.0))
+
+
 def tuned_defaults(tune_entry):
     t = dict(engine=bfn.JACOBI_PULL, block=256, sweeps=0, control=bfn.PERSISTENT,
              classes=None, items_per_claim=1, use_graph=0, graph_rounds=0, boundary_mode=0,
-             min_waves=0, claim_order=0)
+             min_waves=0, claim_order=0, push_window=0.0)
     if tune_entry:
         t.update(tune_entry)
     return t
@@ -306,7 +312,7 @@ def negotiated(dev, tune):
         ws.submit(dg, batch, opts, bfn.tuning(tile_sweeps=tune["sweeps"], count_relaxations=1,
                                               items_per_claim=tune["items_per_claim"],
                                               boundary_mode=tune["boundary_mode"],
-                                              min_waves=tune["min_waves"], claim_order=tune.get("claim_order", 0)))
+                                              min_waves=tune["min_waves"], **scheduling(tune)))
         st, per, _ = ws.status()
         relax_ms = ws.timing()[0]
         relaxations = sum(p.relaxations for p in per)
@@ -428,8 +434,8 @@ def main():
                                          block_size=tune["block"], instrumentation=1),
               bfn.tuning(tile_sweeps=tune["sweeps"], count_relaxations=1,
                          items_per_claim=tune["items_per_claim"], use_graph=tune["use_graph"],
-                         graph_rounds=tune["graph_rounds"],
-                         boundary_mode=tune["boundary_mode"], min_waves=tune["min_waves"], claim_order=tune.get("claim_order", 0)))
+                         graph_rounds=tune["graph_rounds"], boundary_mode=tune["boundary_mode"],
+                         min_waves=tune["min_waves"], **scheduling(tune)))
     st, per, _ = ws.status()
     if not st.converged:
         raise SystemExit("batch did not converge")
@@ -441,7 +447,7 @@ def main():
                                           use_graph=tune["use_graph"],
                                           graph_rounds=tune["graph_rounds"],
                                           boundary_mode=tune["boundary_mode"],
-                                          min_waves=tune["min_waves"], claim_order=tune.get("claim_order", 0)))
+                                          min_waves=tune["min_waves"], **scheduling(tune)))
     elapsed, relax_ms, launches, paths_ms = timed_steps(ws, args.steps, args.warmup, paths=True)
     # one round kernel + one advance kernel per host-polled or replayed round
     round_launches = 1 if tune["control"] == bfn.PERSISTENT and not tune["use_graph"] else launches // 2
@@ -483,6 +489,7 @@ def main():
                    "per_class_engines": tune.get("classes"), "block": tune["block"],
                    "tile_sweeps": tune["sweeps"], "boundary_mode": tune["boundary_mode"],
                    "min_waves": tune["min_waves"], "claim_order": tune["claim_order"],
+                   "push_window": tune["push_window"],
                    "control": bfn.CONTROL_NAMES[tune["control"]] +
                    (f"+hipgraph({tune['graph_rounds']} rounds)" if tune["use_graph"] else ""),
                    "relaxations_per_step": relax_per_step,

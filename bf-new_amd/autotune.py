@@ -74,7 +74,8 @@ def time_candidate(dev, dg, batch, cand, reps):
                           graph_rounds=cand.get("graph_rounds", 0),
                           boundary_mode=cand.get("boundary_mode", 0),
                           min_waves=cand.get("min_waves", 0),
-                          claim_order=cand.get("claim_order", 0))
+                          claim_order=cand.get("claim_order", 0),
+                          push_window=cand.get("push_window", 0.0))
         ws.submit(dg, batch, opts, tune)
         st, _, _ = ws.status()
         if not st.converged:
@@ -98,11 +99,11 @@ def candidates(engines, blocks, quick):
     out = []
     for engine, block in itertools.product(engines, blocks):
         jacobi = engine == bfn.JACOBI_PULL
-        waves = (0, 6, 8) if (not jacobi and block in (256, 512)) else (0,)
+        waves = (0, 6, 7, 8) if (not jacobi and block in (256, 512)) else (0,)
         for sw in ((0, 1) if jacobi else (0,)):
             for mode in ((0,) if jacobi else (0, 1)):
                 for mw in waves:
-                    for ipc in ((2,) if quick else (1, 2)):
+                    for ipc in ((2,) if quick else (1, 2, 3)):
                         out.append(dict(engine=engine, block=block, sweeps=sw, items_per_claim=ipc,
                                         boundary_mode=mode, min_waves=mw,
                                         control=bfn.PERSISTENT))
@@ -225,6 +226,8 @@ def main():
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--blocks", default="128,256,512")
     ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--engines", default="0,1,2", help="engines to try (0 Jacobi pull, 1 dense push, 2 frontier push)")
+    ap.add_argument("--windows", default="2,4,8", help="push windows to try besides none")
     ap.add_argument("--out", default=os.path.join(HERE, "tuning"))
     ap.add_argument("--shootout", action="store_true",
                     help="run the per-query shootout campaign and write its TSV records")
@@ -243,7 +246,8 @@ def main():
         return
 
     blocks = [int(b) for b in args.blocks.split(",")]
-    engines = (bfn.JACOBI_PULL, bfn.DENSE_CHAOTIC_PUSH, bfn.FRONTIER_PUSH)
+    engines = tuple(int(e) for e in args.engines.split(","))
+    windows = [float(w) for w in args.windows.split(",") if float(w) > 0]
     trials = []
     ref_bits = None
 
@@ -264,7 +268,8 @@ def main():
             f"{cand['sweeps']} boundary {cand['boundary_mode']} waves {cand['min_waves']} "
             f"ipc {cand['items_per_claim']} control "
             f"{bfn.CONTROL_NAMES[cand['control']]}{'+graph' if cand.get('use_graph') else ''}"
-            f"{' slot-order' if cand.get('claim_order') else ''}: "
+            f"{' slot-order' if cand.get('claim_order') else ''}"
+            f"{' window ' + str(cand['push_window']) if cand.get('push_window') else ''}: "
             f"{ms:.3f} ms, {rounds} rounds")
         return rounds
 
@@ -278,6 +283,13 @@ def main():
             trial(twin)
             # and in slot order (queue bits compacted between round launches)
             trial(dict(twin, claim_order=1))
+        if rounds is not None and cand["engine"] != bfn.JACOBI_PULL:
+            # Push windows: a persistent run gives the round count to capture.
+            for w in windows:
+                wr = trial(dict(cand, push_window=w))
+                if wr is not None:
+                    trial(dict(cand, push_window=w, control=bfn.CHUNKED, use_graph=1,
+                               graph_rounds=wr + 2))
 
     for cand in candidates(engines, blocks, args.quick):
         with_graph_twin(cand)
@@ -306,6 +318,7 @@ def main():
     entry = {k: best[k] for k in ("engine", "block", "sweeps", "control", "items_per_claim",
                                   "boundary_mode", "min_waves")}
     entry["claim_order"] = best.get("claim_order", 0)
+    entry["push_window"] = best.get("push_window", 0.0)
     entry["use_graph"] = best.get("use_graph", 0)
     entry["graph_rounds"] = best.get("graph_rounds", 0)
     entry["ms"] = best["ms"]
@@ -333,7 +346,7 @@ def main():
         json.dump(table, f, indent=1, sort_keys=True)
     with open(os.path.join(args.out, f"trials_{args.workload}.tsv"), "w") as f:
         cols = ["scope", "engine", "block", "sweeps", "boundary_mode", "min_waves", "items_per_claim",
-                "control", "use_graph", "graph_rounds", "claim_order", "rounds", "ms"]
+                "control", "use_graph", "graph_rounds", "claim_order", "push_window", "rounds", "ms"]
         f.write("\t".join(cols) + "\n")
         for t in trials:
             f.write("\t".join(str(t.get(c, 0)) for c in cols) + "\n")

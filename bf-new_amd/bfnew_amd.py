@@ -79,7 +79,8 @@ class Tuning(C.Structure):
     _fields_ = [("tile_sweeps", C.c_uint32), ("items_per_claim", C.c_uint32),
                 ("graph_rounds", C.c_uint32), ("use_graph", C.c_uint32),
                 ("count_relaxations", C.c_uint32), ("boundary_mode", C.c_uint32),
-                ("min_waves", C.c_uint32), ("claim_order", C.c_uint32)]
+                ("min_waves", C.c_uint32), ("claim_order", C.c_uint32),
+                ("push_window", C.c_float)]
 
 
 _P32 = C.POINTER(C.c_uint32)
@@ -304,12 +305,13 @@ def run_options(engine=JACOBI_PULL, control_mode=PERSISTENT, rounds_per_chunk=8,
 
 
 def tuning(tile_sweeps=1, items_per_claim=1, graph_rounds=0, use_graph=0, count_relaxations=0,
-           boundary_mode=0, min_waves=0, claim_order=0):
+           boundary_mode=0, min_waves=0, claim_order=0, push_window=0.0):
     t = Tuning()
     hip_lib().bfn_tuning_default(C.byref(t))
     t.tile_sweeps, t.items_per_claim = tile_sweeps, items_per_claim
     t.graph_rounds, t.use_graph, t.count_relaxations = graph_rounds, use_graph, count_relaxations
     t.boundary_mode, t.min_waves, t.claim_order = boundary_mode, min_waves, claim_order
+    t.push_window = push_window
     return t
 
 

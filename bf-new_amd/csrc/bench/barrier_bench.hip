@@ -27,8 +27,11 @@ __global__ __launch_bounds__(256) void barrier_kernel(DevControl* c, std::uint32
   unsigned long long acc = blockIdx.x + 1ull;
   for (std::uint32_t r = 0; r < rounds; ++r) {
     acc = acc * 6364136223846793005ull + r + threadIdx.x;
-    if constexpr (Kind == 0) bfn::grid_barrier_flat(c, gridDim.x);
-    else if constexpr (Kind == 1) bfn::grid_barrier_grouped(c, gridDim.x);
+    if constexpr (Kind == 0) {
+      if (!bfn::grid_barrier_flat(c, gridDim.x)) break;
+    } else if constexpr (Kind == 1) {
+      if (!bfn::grid_barrier_grouped(c, gridDim.x)) break;
+    }
     else cooperative_groups::this_grid().sync();
   }
   if (threadIdx.x == 0) sink[blockIdx.x] = acc;

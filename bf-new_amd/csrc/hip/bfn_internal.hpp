@@ -54,7 +54,14 @@ constexpr int kPhaseCount = 5;  // visit phases timed by instrumented runs
 // (rounded up to 4): neighbour slots [0, K), pushed-into marks [K, 2K),
 // per-neighbour offsets [2K, 4K) (two words each: a pool offset, or one
 // word: a staged slab offset); then the flag words.
-constexpr std::uint32_t kLdsFlagWords = 8;
+constexpr std::uint32_t kLdsFlagWords = 16;
+
+// Relaxation kernel variants by the engines of a batch: a kernel that only
+// holds the push visit (or only the pull visit) needs fewer registers per
+// lane than one that holds both, so more waves are resident.
+constexpr std::uint32_t kEnginesMixed = 0;
+constexpr std::uint32_t kEnginesPush = 1;
+constexpr std::uint32_t kEnginesPull = 2;
 // Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8), so
 // a barrier that first gathers each XCD's workgroups spreads its arrivals
 // over 8 counters instead of one.
@@ -139,6 +146,13 @@ struct DevControl {
   // position of segment x, each counter on its own 128-byte line.
   std::uint32_t seg_begin[kBarrierGroups + 1];
   alignas(128) std::uint32_t seg_claim[kBarrierGroups * kBarrierStride];
+  // Push window: rounds visit the queued slots whose priority (smallest
+  // label pushed in since their last visit) is at most `window_limit`.
+  // Round r gathers the smallest priority it leaves queued for round r + 1
+  // in next_min[(r+1) % 3] (one counter per XCD group, each on its own
+  // line); the limit of round r + 1 is that minimum plus the window.
+  std::uint32_t window_limit;
+  alignas(128) std::uint32_t next_min[3 * kBarrierGroups * kBarrierStride];
   // Arrival counters of the two-level grid barrier, one per XCD group of
   // workgroups (blockIdx % 8), each on its own 128-byte line.
   alignas(128) std::uint32_t group_arrivals[kBarrierGroups * kBarrierStride];
@@ -169,6 +183,10 @@ struct DevBatch {
   std::uint32_t boundary_mode;    // push: 0 LDS proposals when they fit, 1 direct atomics
   std::uint32_t min_waves;        // kernel variant bounded to >= this many waves per SIMD
   std::uint32_t ordered;          // slot-order claims: queue as bits in `active`
+  std::uint32_t window;           // push window (float bits), 0 = off
+  std::uint32_t engines;          // kEnginesMixed / kEnginesPush / kEnginesPull: the
+                                  // relaxation kernel variant, by the engines the batch holds
+  std::uint32_t probe;            // persistent kernel: pass one grid barrier and exit
   const std::uint32_t* tile_begin;      // [Q]   first position of q in query_tiles
   const std::uint32_t* tile_end;        // [Q]   one past q's last position
   const std::uint32_t* query_tiles;     // [.]   q's selected tiles, sorted
@@ -194,6 +212,7 @@ struct DevBatch {
   // (ceil(S/32) words, sharing the stamp buffer, which that mode leaves
   // unused); the compacted list of a round is lists[0 .. n).
   std::uint32_t* active;
+  std::uint32_t* prio;                  // [S] push window: priority of each slot, or null
   std::uint32_t* lists;                 // [3*S]
   std::uint32_t* dist;                  // float bits pool
   std::uint32_t* last_change;           // [Q] 1 + last round with a decrease
@@ -256,6 +275,9 @@ __device__ __forceinline__ std::uint32_t atomic_max_agent(std::uint32_t* p, std:
 }
 __device__ __forceinline__ std::uint32_t atomic_add_agent(std::uint32_t* p, std::uint32_t v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ std::uint32_t atomic_exch_agent(std::uint32_t* p, std::uint32_t v) {
+  return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void atomic_or_agent(std::uint32_t* p, std::uint32_t v) {
   __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -322,12 +344,33 @@ __host__ __device__ constexpr std::size_t relax_lds_words(std::uint32_t max_v, s
 }
 
 // Grid barriers of the persistent kernels (every workgroup of the grid
-// calls them the same number of times; the grid is co-resident).
-//
+// calls them the same number of times; the grid is co-resident).  A wait is
+// bounded: after kBarrierSpinLimit polls (seconds, far beyond any round) the
+// workgroup records kErrorBarrierTimeout and the barrier returns false, so a
+// grid that is not co-resident ends the run with a device failure instead
+// of holding the GPU.
+constexpr std::uint32_t kBarrierSpinLimit = 1u << 23;
+constexpr std::uint32_t kProbeSpinLimit = 1u << 16;  // co-residency probe launches
+constexpr std::uint32_t kErrorBarrierTimeout = 8u;
+
+__device__ __forceinline__ bool barrier_wait(DevControl* c, std::uint32_t gen, std::uint32_t limit) {
+  for (std::uint32_t spins = 0; load_agent(&c->barrier_generation) == gen; ++spins) {
+    if (spins == limit) {
+      __hip_atomic_fetch_or(&c->error_bits, kErrorBarrierTimeout, __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return true;
+}
+
 // Flat: every workgroup arrives on one counter; the last arriver resets it
 // and advances the generation that the others wait on.
-__device__ __forceinline__ void grid_barrier_flat(DevControl* c, std::uint32_t blocks) {
+__device__ __forceinline__ bool grid_barrier_flat(DevControl* c, std::uint32_t blocks,
+                                                  std::uint32_t limit = kBarrierSpinLimit) {
   __syncthreads();
+  bool ok = true;
   if (threadIdx.x == 0) {
     const std::uint32_t gen = load_agent(&c->barrier_generation);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -337,11 +380,11 @@ __device__ __forceinline__ void grid_barrier_flat(DevControl* c, std::uint32_t b
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
       store_agent(&c->barrier_generation, gen + 1);
     } else {
-      while (load_agent(&c->barrier_generation) == gen) __builtin_amdgcn_s_sleep(1);
+      ok = barrier_wait(c, gen, limit);
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
-  __syncthreads();
+  return __syncthreads_or(!ok) == 0;
 }
 
 // Two-level: a workgroup arrives on its group's counter (group =
@@ -350,8 +393,10 @@ __device__ __forceinline__ void grid_barrier_flat(DevControl* c, std::uint32_t b
 // arrive resets the global counter and advances the generation.  Every
 // counter is reset before the generation moves, so the next barrier's
 // arrivals always find zeros.
-__device__ __forceinline__ void grid_barrier_grouped(DevControl* c, std::uint32_t blocks) {
+__device__ __forceinline__ bool grid_barrier_grouped(DevControl* c, std::uint32_t blocks,
+                                                     std::uint32_t limit = kBarrierSpinLimit) {
   __syncthreads();
+  bool ok = true;
   if (threadIdx.x == 0) {
     const std::uint32_t gen = load_agent(&c->barrier_generation);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -370,11 +415,11 @@ __device__ __forceinline__ void grid_barrier_grouped(DevControl* c, std::uint32_
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
       store_agent(&c->barrier_generation, gen + 1);
     } else {
-      while (load_agent(&c->barrier_generation) == gen) __builtin_amdgcn_s_sleep(1);
+      ok = barrier_wait(c, gen, limit);
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
-  __syncthreads();
+  return __syncthreads_or(!ok) == 0;
 }
 
 // Position of key in the sorted range values[begin, end), or kNone.
@@ -431,6 +476,23 @@ __device__ __forceinline__ std::uint32_t find_slot(const DevBatch& b, std::uint3
                                                    std::uint32_t tile) {
   const std::uint32_t pos = find_sorted(b.query_tiles, b.tile_begin[q], b.tile_end[q], tile);
   return pos == kNone ? kNone : slot_at(b, pos);
+}
+
+// Push window bookkeeping between rounds.  The limit of round r + 1 is the
+// smallest priority round r left queued plus the window (never below the
+// limit of round r), so the slot holding that priority is always visited
+// in round r + 1 and every queued slot is visited eventually.
+__device__ __forceinline__ std::uint32_t next_window_limit(DevControl* c, const DevBatch& b,
+                                                           std::uint32_t r, std::uint32_t limit) {
+  std::uint32_t m = kInfBits;
+  for (std::uint32_t x = 0; x < kBarrierGroups; ++x)
+    m = min(m, load_agent(&c->next_min[(((r + 1) % 3) * kBarrierGroups + x) * kBarrierStride]));
+  if (m == kInfBits) return limit;
+  return max(limit, as_bits(as_float(m) + as_float(b.window)));
+}
+__device__ __forceinline__ void reset_next_min(DevControl* c, std::uint32_t buffer) {
+  for (std::uint32_t x = 0; x < kBarrierGroups; ++x)
+    store_agent(&c->next_min[(buffer * kBarrierGroups + x) * kBarrierStride], kInfBits);
 }
 
 }  // namespace bfn

@@ -40,10 +40,33 @@ hipError_t relax_launch(std::uint32_t block, bool persistent, bool cooperative,
                         std::uint32_t grid, std::size_t lds_bytes, std::uint32_t max_v,
                         std::uint32_t work_words, const DevGraph& g, const DevBatch& b,
                         hipStream_t stream);
-hipError_t relax_occupancy(std::uint32_t block, bool persistent, bool instrumented,
-                           std::uint32_t min_waves, std::size_t lds_bytes, int* blocks_per_cu);
+hipError_t relax_occupancy(std::uint32_t engines, std::uint32_t block, bool persistent,
+                           bool instrumented, std::uint32_t min_waves, std::size_t lds_bytes,
+                           int* blocks_per_cu);
+// The kernel variants of one engine set (relax_mixed.hip, relax_push.hip,
+// relax_pull.hip); relax_launch and relax_occupancy pick by engine set.
+#define BFN_RELAX_ENGINE_SET(name)                                                             \
+  hipError_t relax_launch_##name(std::uint32_t block, bool persistent, bool cooperative,      \
+                                 std::uint32_t grid, std::size_t lds_bytes, std::uint32_t max_v, \
+                                 std::uint32_t work_words, const DevGraph& g, const DevBatch& b, \
+                                 hipStream_t stream);                                          \
+  hipError_t relax_occupancy_##name(std::uint32_t block, bool persistent, bool instrumented,  \
+                                    std::uint32_t min_waves, std::size_t lds_bytes,            \
+                                    int* blocks_per_cu);
+BFN_RELAX_ENGINE_SET(mixed)
+BFN_RELAX_ENGINE_SET(push)
+BFN_RELAX_ENGINE_SET(pull)
+#undef BFN_RELAX_ENGINE_SET
 hipError_t prep_launch(const DevGraph& g, const DevBatch& b, std::uint64_t pool_words,
                        hipStream_t stream);
+// Workgroups per CU (at most `promised`) whose persistent grid the device
+// actually places at once, checked by a probe launch of the variant that `b`
+// selects (b.engines, b.min_waves, b.instrumentation; b.control is reset and
+// used); 0 when not even one per CU passes.
+hipError_t coresident_per_cu(int device, int promised, std::uint32_t cus, std::size_t lds_bytes,
+                             std::uint32_t max_v, std::uint32_t work_words, const DevGraph& g,
+                             const DevBatch& b, hipStream_t stream, std::uint32_t block,
+                             int* placed);
 // Inputs and outputs of the tile-image derivation (graph.hip).
 struct TileImageArgs {
   std::uint32_t tile_count;

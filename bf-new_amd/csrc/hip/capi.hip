@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -160,7 +161,7 @@ struct bfn_workspace_s {
   hipStream_t stream = nullptr;
   std::uint64_t budget = 0;
   // batch metadata in one device buffer + separately grown state
-  DeviceBuffer meta, slot_in, slot_out, slot_out_off, slot_desc, stamp, lists, pool, last_change, control,
+  DeviceBuffer meta, slot_in, slot_out, slot_out_off, slot_desc, stamp, prio, lists, pool, last_change, control,
       status, relax,
       scratch_v, paths_summary, paths_len, paths_off, frames, onpath, clean, arena_v, arena_l,
       arena_e, occupancy, region_off;
@@ -793,7 +794,7 @@ std::size_t meta_layout(std::uint32_t Q, std::uint32_t S, std::uint32_t ns, std:
 
 std::uint64_t batch_bytes(const BatchPlanHost& p, std::uint64_t meta_bytes) {
   return meta_bytes + 4ull * (p.in_total + p.out_total) + 8ull * p.out_total +
-         (16ull + sizeof(SlotDesc)) * p.S +
+         (20ull + sizeof(SlotDesc)) * p.S +
          4ull * p.pool_words + 4ull * p.Q * 2 + sizeof(QueryStatusDev) * p.Q + 8ull * p.Q +
          sizeof(DevControl);
 }
@@ -812,6 +813,7 @@ void bind_view(bfn_workspace ws, const BatchPlanHost& p, const std::size_t* seg,
   b.boundary_mode = t->boundary_mode;
   b.min_waves = t->min_waves;
   b.ordered = t->claim_order;
+  std::memcpy(&b.window, &t->push_window, sizeof b.window);
   b.tile_begin = reinterpret_cast<const std::uint32_t*>(m + seg[0]);
   b.tile_end = b.tile_begin + 1;
   b.query_tiles = reinterpret_cast<const std::uint32_t*>(m + seg[1]);
@@ -832,6 +834,7 @@ void bind_view(bfn_workspace ws, const BatchPlanHost& p, const std::size_t* seg,
   b.slot_desc = ws->slot_desc.as<SlotDesc>();
   b.stamp = ws->stamp.as<std::uint32_t>();
   b.active = b.stamp;
+  b.prio = b.window ? ws->prio.as<std::uint32_t>() : nullptr;
   b.lists = ws->lists.as<std::uint32_t>();
   b.dist = ws->pool.as<std::uint32_t>();
   b.last_change = ws->last_change.as<std::uint32_t>();
@@ -991,6 +994,8 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
       t.claim_order > 1)
     return fail(BFN_E_INVALID_ARGUMENT,
                 "invalid tuning (tile_sweeps, boundary_mode, use_graph, min_waves, claim_order)");
+  if (!(t.push_window >= 0.0f) || !std::isfinite(t.push_window))
+    return fail(BFN_E_INVALID_ARGUMENT, "push_window must be a finite non-negative number");
   if (t.claim_order && o->control_mode == 0)
     return fail(BFN_E_INVALID_ARGUMENT,
                 "claim_order 1 compacts the queue between round launches: it needs per-round "
@@ -1029,7 +1034,8 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
                                              std::to_string(lds_bytes) + " > " +
                                              std::to_string(lds_limit) + " bytes)");
   int per_cu = 0;
-  BFN_HIP(relax_occupancy(o->block_size, o->control_mode == 0, o->instrumentation != 0,
+  const std::uint32_t engines = !p.any_jacobi ? kEnginesPush : !p.any_push ? kEnginesPull : kEnginesMixed;
+  BFN_HIP(relax_occupancy(engines, o->block_size, o->control_mode == 0, o->instrumentation != 0,
                           t.min_waves, lds_bytes, &per_cu));
   if (per_cu <= 0) return fail(BFN_E_NOT_REPRESENTABLE, "relaxation kernel cannot be resident");
   const std::uint32_t cus = static_cast<std::uint32_t>(ws->device->props.multiProcessorCount);
@@ -1059,6 +1065,7 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
   BFN_HIP(ws->slot_out_off.reserve(8ull * std::max<std::uint64_t>(p.out_total, 1)));
   BFN_HIP(ws->slot_desc.reserve(sizeof(SlotDesc) * std::max(S, 1u)));
   BFN_HIP(ws->stamp.reserve(4ull * std::max(S, 1u)));
+  if (t.push_window > 0.0f) BFN_HIP(ws->prio.reserve(4ull * std::max(S, 1u)));
   BFN_HIP(ws->lists.reserve(12ull * std::max(S, 1u)));
   BFN_HIP(ws->pool.reserve(4ull * std::max<std::uint64_t>(p.pool_words, 1)));
   BFN_HIP(ws->last_change.reserve(4ull * Q));
@@ -1084,6 +1091,17 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
   bind_view(ws, p, seg, o, &t);
   ws->frame_base_ptr = reinterpret_cast<const unsigned long long*>(ws->meta.as<char>() + seg[13]);
   if (!proposals) ws->view.boundary_mode = 1;
+  ws->view.engines = engines;
+  if (o->control_mode == 0) {
+    // A persistent grid must be co-resident: checked on the device.
+    int placed = 0;
+    BFN_HIP(coresident_per_cu(ws->device->ordinal, per_cu, cus, lds_bytes, max_v, work_words,
+                              graph->view, ws->view, ws->stream, o->block_size, &placed));
+    if (placed <= 0)
+      return fail(BFN_E_NOT_REPRESENTABLE, "no persistent grid of the relaxation kernel is co-resident");
+    blocks = static_cast<std::uint32_t>(placed) * cus;
+    if (o->grid_policy == 1) blocks = std::min(blocks, o->blocks_per_wgp * cus);
+  }
   ws->block = o->block_size;
   ws->max_v = max_v;
   ws->work_words = work_words;

@@ -21,6 +21,7 @@
 #include "bfn_launch.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -158,6 +159,8 @@ int32_t bfn_batch_run_static(bfn_graph graph, const bfn_batch_desc* d, const bfn
   if (t.tile_sweeps > 1 || t.boundary_mode > 1 || t.min_waves > 8 || t.claim_order > 1)
     return fail(BFN_E_INVALID_ARGUMENT,
                 "invalid tuning (tile_sweeps, boundary_mode, min_waves, claim_order)");
+  if (!(t.push_window >= 0.0f) || !std::isfinite(t.push_window))
+    return fail(BFN_E_INVALID_ARGUMENT, "push_window must be a finite non-negative number");
   if (t.claim_order && o->control_mode == 0)
     return fail(BFN_E_INVALID_ARGUMENT, "claim_order 1 needs per-round control (control_mode 1 or 2)");
   if (t.use_graph || t.count_relaxations)
@@ -209,6 +212,7 @@ int32_t bfn_batch_run_static(bfn_graph graph, const bfn_batch_desc* d, const bfn
   const std::size_t o_slot_in = take(4ull * in_total);
   const std::size_t o_slot_out = take(4ull * out_total);
   const std::size_t o_stamp = take(4ull * S);
+  const std::size_t o_prio = take(t.push_window > 0.0f ? 4ull * S : 0ull);
   const std::size_t o_lists = take(12ull * S);
   const std::size_t o_last = take(4ull * Q);
   const std::size_t o_pool = take(4ull * pool_words);
@@ -228,7 +232,8 @@ int32_t bfn_batch_run_static(bfn_graph graph, const bfn_batch_desc* d, const bfn
   const bool persistent = o->control_mode == 0;
   int per_cu = 0;
   BFN_HIP_STATIC(hipSetDevice(gh.ordinal));
-  BFN_HIP_STATIC(relax_occupancy(o->block_size, persistent, o->instrumentation != 0, t.min_waves,
+  const std::uint32_t engines = !any_jacobi ? kEnginesPush : !any_push ? kEnginesPull : kEnginesMixed;
+  BFN_HIP_STATIC(relax_occupancy(engines, o->block_size, persistent, o->instrumentation != 0, t.min_waves,
                                  lds_bytes, &per_cu));
   if (per_cu <= 0) return fail(BFN_E_NOT_REPRESENTABLE, "relaxation kernel cannot be resident");
   std::uint32_t grid = static_cast<std::uint32_t>(per_cu) * gh.cu_count;
@@ -249,6 +254,8 @@ int32_t bfn_batch_run_static(bfn_graph graph, const bfn_batch_desc* d, const bfn
   b.boundary_mode = 1;
   b.min_waves = t.min_waves;
   b.ordered = t.claim_order;
+  b.engines = engines;
+  std::memcpy(&b.window, &t.push_window, sizeof b.window);
   b.tile_begin = u32(o_tile_off);
   b.tile_end = u32(o_tile_off) + 1;
   b.query_tiles = u32(o_tiles);
@@ -269,6 +276,7 @@ int32_t bfn_batch_run_static(bfn_graph graph, const bfn_batch_desc* d, const bfn
   b.slot_desc = reinterpret_cast<const SlotDesc*>(m + o_desc);
   b.stamp = u32(o_stamp);
   b.active = u32(o_stamp);
+  b.prio = b.window ? u32(o_prio) : nullptr;
   b.lists = u32(o_lists);
   b.dist = u32(o_pool);
   b.last_change = u32(o_last);
@@ -278,6 +286,16 @@ int32_t bfn_batch_run_static(bfn_graph graph, const bfn_batch_desc* d, const bfn
   const hipStream_t s = gh.stream;
   // Nothing of an earlier call may still read the scratch.
   BFN_HIP_STATIC(hipStreamSynchronize(s));
+  if (persistent) {
+    // A persistent grid must be co-resident: checked on the device.
+    int placed = 0;
+    BFN_HIP_STATIC(coresident_per_cu(gh.ordinal, per_cu, gh.cu_count, lds_bytes, max_v, work_words,
+                                     gv, b, s, o->block_size, &placed));
+    if (placed <= 0)
+      return fail(BFN_E_NOT_REPRESENTABLE, "no persistent grid of the relaxation kernel is co-resident");
+    grid = static_cast<std::uint32_t>(placed) * gh.cu_count;
+    if (o->grid_policy == 1) grid = std::min(grid, o->blocks_per_wgp * gh.cu_count);
+  }
   auto h2d = [&](std::size_t off, const void* src, std::size_t bytes) {
     return bytes ? hipMemcpyAsync(m + off, src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
   };

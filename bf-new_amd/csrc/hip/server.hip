@@ -576,6 +576,9 @@ int32_t bfn_server_create(bfn_device device, bfn_graph graph, const bfn_server_o
   b.min_waves = 0;
   b.ordered = 0;  // arrival-order lists: admitted work joins the running round
   b.active = nullptr;
+  b.window = 0;
+  b.prio = nullptr;
+  b.engines = o->engine == kJacobiPull ? kEnginesPull : kEnginesPush;  // one engine per server
   // A request's slots form one contiguous range of the arena, in tile
   // order, so positions and slots coincide.
   b.tile_begin = sv->d_begin;
@@ -605,7 +608,7 @@ int32_t bfn_server_create(bfn_device device, bfn_graph graph, const bfn_server_o
                      sv->Scap, sv->slab_words, sv->kin, sv->kout, sv->d_active, sv->d_src_off,
                      sv->d_tgt_off, sv->d_clean);
   int per_cu = 0;
-  if (relax_occupancy(sv->block, false, false, 0, sv->lds_bytes, &per_cu) != hipSuccess ||
+  if (relax_occupancy(b.engines, sv->block, false, false, 0, sv->lds_bytes, &per_cu) != hipSuccess ||
       per_cu <= 0)
     return bail(BFN_E_NOT_REPRESENTABLE, "relaxation kernel cannot be resident");
   sv->grid = static_cast<std::uint32_t>(per_cu) * device_cu_count(device);

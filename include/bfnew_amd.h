@@ -137,15 +137,22 @@ typedef struct {
                                     1 = per boundary edge, a read of the
                                     neighbour's label, then a global atomic
                                     when the candidate is below it */
-  uint32_t min_waves;            /* 0 = compiler default; 6 or 8 = the 256- and
-                                    512-thread kernels compiled for at least that
-                                    many waves per SIMD */
+  uint32_t min_waves;            /* 0 = compiler default; 6, 7 or 8 = the 256-
+                                    and 512-thread kernels compiled for at least
+                                    that many waves per SIMD */
   uint32_t claim_order;          /* per-round launches only: 0 = work items in
                                     arrival order (one list per round); 1 = work
                                     items in slot order (tile-major), split into
                                     eight segments that the workgroups of each
                                     XCD claim first (queued as bits, compacted
                                     between rounds) */
+  float push_window;             /* push engines: 0 = a round visits every queued
+                                    slot; w > 0 = a round visits only the queued
+                                    slots whose smallest label pushed into them
+                                    is at most m + w, m the smallest such label
+                                    over the queue (the others stay queued for a
+                                    later round): fewer repeated visits, more
+                                    rounds.  Jacobi pull ignores it. */
 } bfn_tuning;
 
 /* The reference DeviceGraphLayout32 (device_layout.hpp:72-93) plus the tile

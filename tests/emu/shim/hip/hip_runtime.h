@@ -35,6 +35,7 @@
 #define __hip_atomic_fetch_min emu__hip_atomic_fetch_min
 #define __hip_atomic_fetch_max emu__hip_atomic_fetch_max
 #define __hip_atomic_fetch_or emu__hip_atomic_fetch_or
+#define __hip_atomic_exchange emu__hip_atomic_exchange
 #define __HIP_MEMORY_SCOPE_WORKGROUP 0
 
 struct dim3 {
@@ -159,6 +160,7 @@ template <typename T> inline T emu__hip_atomic_fetch_add(T* p, T v, int, int) { 
 template <typename T> inline T emu__hip_atomic_fetch_min(T* p, T v, int, int) { return atomicMin(p, v); }
 template <typename T> inline T emu__hip_atomic_fetch_max(T* p, T v, int, int) { return atomicMax(p, v); }
 template <typename T> inline T emu__hip_atomic_fetch_or(T* p, T v, int, int) { return atomicOr(p, v); }
+template <typename T> inline T emu__hip_atomic_exchange(T* p, T v, int, int) { return std::atomic_ref<T>(*p).exchange(v); }
 
 // ---- runtime API ----------------------------------------------------------
 enum hipError_t { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorOutOfMemory = 2, hipErrorNotReady = 600 };

@@ -310,6 +310,47 @@ def test_slot_order_claims(device):
     assert refused.value.code == 1
 
 
+def test_push_window(device):
+    """push_window w > 0 (a round visits only the queued push slots whose
+    smallest pushed-in label is within w of the queue's smallest; the others
+    stay queued): both push engines, persistent, per-round and chunked
+    control with hipGraph replay (arrival and slot order), narrow and wide
+    windows, bit-exact against the oracle on two runs; a mixed batch keeps
+    its Jacobi queries on reference rounds; a negative or non-finite window
+    is refused."""
+    hg, dg = synthetic(device, 32, 32, 128, 6)
+    view = hg.view()
+    qs, batch = c2_batch(hg, 12, seed=17)
+    refs = [oracle_query(view, [q.source], batch.tiles[i])[0] for i, q in enumerate(qs)]
+    controls = ((bfn.PERSISTENT, 0, 0), (bfn.PER_ROUND, 0, 0), (bfn.CHUNKED, 1, 0), (bfn.CHUNKED, 1, 1))
+    for eng in (bfn.DENSE_CHAOTIC_PUSH, bfn.FRONTIER_PUSH):
+        for ctl, graph, order in controls:
+            for window in (0.25, 4.0):
+                ws = run(device, dg, batch, eng, ctl, 0, items_per_claim=2, use_graph=graph,
+                         graph_rounds=6 if graph else 0, boundary_mode=1, claim_order=order,
+                         push_window=window)
+                for rep in range(2):
+                    assert ws.status()[0].converged == 1, (eng, ctl, order, window, rep)
+                    for i in range(len(qs)):
+                        assert np.array_equal(bits(ws.distances(i)), bits(refs[i])), (eng, ctl, order, window, rep, i)
+                    ws.rerun()
+                ws.close()
+    batch.engines = [bfn.JACOBI_PULL if i % 2 else bfn.FRONTIER_PUSH for i in range(len(qs))]
+    ws = run(device, dg, batch, bfn.FRONTIER_PUSH, bfn.PERSISTENT, 1, push_window=1.0)
+    alone = run(device, dg, bfn.Batch([[qs[1].source]], [[qs[1].target]], [batch.tiles[1]]),
+                bfn.JACOBI_PULL, bfn.PERSISTENT, 1)
+    assert ws.status()[1][1].rounds_completed == alone.status()[1][0].rounds_completed
+    for i in range(len(qs)):
+        assert np.array_equal(bits(ws.distances(i)), bits(refs[i])), i
+    ws.close()
+    alone.close()
+    batch.engines = None
+    for bad in (-1.0, float("inf"), float("nan")):
+        with pytest.raises(bfn.BfnError) as refused:
+            run(device, dg, batch, bfn.FRONTIER_PUSH, bfn.PERSISTENT, 0, push_window=bad)
+        assert refused.value.code == 1
+
+
 def test_relaxation_counts_match_oracle(device):
     hg, dg = synthetic(device, 32, 32, 128, 8)
     view = hg.view()
