@@ -93,7 +93,7 @@ def time_candidate(dev, dg, batch, cand, reps):
         ws.close()
 
 
-def candidates(engines, blocks, quick):
+def candidates(engines, blocks, quick, boundary_modes=(0, 1)):
     """Persistent candidates; each is followed by its hipGraph twin, whose
     captured round count is set from the persistent run."""
     out = []
@@ -101,7 +101,7 @@ def candidates(engines, blocks, quick):
         jacobi = engine == bfn.JACOBI_PULL
         waves = (0, 6, 7, 8) if (not jacobi and block in (256, 512)) else (0,)
         for sw in ((0, 1) if jacobi else (0,)):
-            for mode in ((0,) if jacobi else (0, 1)):
+            for mode in ((0,) if jacobi else boundary_modes):
                 for mw in waves:
                     for ipc in ((2,) if quick else (1, 2, 3)):
                         out.append(dict(engine=engine, block=block, sweeps=sw, items_per_claim=ipc,
@@ -228,6 +228,7 @@ def main():
     ap.add_argument("--quick", action="store_true")
     ap.add_argument("--engines", default="0,1,2", help="engines to try (0 Jacobi pull, 1 dense push, 2 frontier push)")
     ap.add_argument("--windows", default="2,4,8", help="push windows to try besides none")
+    ap.add_argument("--boundary-modes", default="0,1", help="push boundary modes to try")
     ap.add_argument("--out", default=os.path.join(HERE, "tuning"))
     ap.add_argument("--shootout", action="store_true",
                     help="run the per-query shootout campaign and write its TSV records")
@@ -291,7 +292,8 @@ def main():
                     trial(dict(cand, push_window=w, control=bfn.CHUNKED, use_graph=1,
                                graph_rounds=wr + 2))
 
-    for cand in candidates(engines, blocks, args.quick):
+    modes = tuple(int(m) for m in args.boundary_modes.split(","))
+    for cand in candidates(engines, blocks, args.quick, modes):
         with_graph_twin(cand)
     best = min(trials, key=lambda t: t["ms"])
 
