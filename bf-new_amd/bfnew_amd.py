@@ -507,6 +507,7 @@ class DeviceGraph:
     def __init__(self, device: Device, layout: GraphLayout32):
         self.device = device
         self.vertex_count = layout.vertex_count
+        self.edge_count = layout.edge_count
         self.h = C.c_void_p()
         _check(hip_lib().bfn_graph_upload(device.h, C.byref(layout), C.byref(self.h)))
 
@@ -527,6 +528,8 @@ class DeviceGraph:
     def update_weights(self, csr_weights, csc_weights):
         a = np.ascontiguousarray(csr_weights, np.float32)
         b = np.ascontiguousarray(csc_weights, np.float32)
+        if a.shape != (self.edge_count,) or b.shape != (self.edge_count,):
+            raise ValueError(f"update_weights: both orientations need {self.edge_count} weights")
         _check(hip_lib().bfn_graph_update_weights(self.h, _ptr(a), _ptr(b)))
 
     def close(self):

@@ -535,10 +535,29 @@ int32_t bfn_graph_query(bfn_graph graph, bfn_graph_info* out) {
 int32_t bfn_graph_update_weights(bfn_graph graph, const float* csr_weights,
                                  const float* csc_weights) {
   if (!graph || !csr_weights || !csc_weights) return fail(BFN_E_INVALID_ARGUMENT, "null argument");
-  for (std::uint32_t e = 0; e < graph->E; ++e)
+  // The two orientations hold the same edges in different orders: push
+  // engines read the CSR weights, pull and path reconstruction the CSC
+  // weights.  Their multisets must agree: compared through order-free exact
+  // sums of the weights' bit patterns and of their squares (mod 2^64) and
+  // the zero counts -- checked in O(E) without matching the edges.
+  std::uint64_t csr_sum = 0, csc_sum = 0, csr_sq = 0, csc_sq = 0, csr_zero = 0, csc_zero = 0;
+  for (std::uint32_t e = 0; e < graph->E; ++e) {
     if (!(csr_weights[e] >= 0.0f) || !(csc_weights[e] >= 0.0f) || csr_weights[e] > 3.4e38f ||
         csc_weights[e] > 3.4e38f)
       return fail(BFN_E_INVALID_ARGUMENT, "weights must be finite and nonnegative");
+    std::uint32_t a, b;
+    std::memcpy(&a, &csr_weights[e], 4);
+    std::memcpy(&b, &csc_weights[e], 4);
+    csr_sum += a;
+    csc_sum += b;
+    csr_sq += static_cast<std::uint64_t>(a) * a;
+    csc_sq += static_cast<std::uint64_t>(b) * b;
+    csr_zero += a == 0u;
+    csc_zero += b == 0u;
+  }
+  if (csr_sum != csc_sum || csr_sq != csc_sq || csr_zero != csc_zero)
+    return fail(BFN_E_INVALID_ARGUMENT, "CSR and CSC weights do not hold the same edge weights");
+  graph->info.has_zero_weight = csr_zero != 0;
   BFN_HIP(hipSetDevice(graph->device->ordinal));
   BFN_HIP(hipMemcpyAsync(graph->w.ptr, csr_weights, sizeof(float) * graph->E,
                          hipMemcpyHostToDevice, graph->stream));

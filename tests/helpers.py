@@ -65,48 +65,6 @@ def oracle_query(view, sources, tiles):
                            view["weights"], view["owner_tiles"], sel, sources)
 
 
-def certify_sssp(layout, sources, tiles, dist):
-    """Size-independent exactness certificate for strictly positive weights:
-    sources are 0, no admitted edge can lower a label (fixed point), and every
-    finite non-source label has a tight admitted in-edge from a finite label.
-    Following tight edges strictly decreases the label, so it ends at a
-    source: the labels are path sums no larger than any admitted path, i.e.
-    exactly the shortest-path fixed point.  Returns a description of the first
-    violation or ''."""
-    owner = layout["owner_tiles"]
-    sel = np.zeros(layout["tile_count"], bool)
-    sel[np.asarray(tiles, np.int64)] = True
-    vsel = sel[owner]
-    d = np.asarray(dist, np.float32)
-    if np.any(np.isfinite(d) & ~vsel):
-        return "finite label outside the selected tiles"
-    srcs = np.asarray(sources, np.int64)
-    if np.any(d[srcs] != 0):
-        return "source label is not zero"
-    if "_csc_dst" not in layout:
-        col = layout["csc_column_offsets"].astype(np.int64)
-        layout["_csc_dst"] = np.repeat(np.arange(len(col) - 1, dtype=np.int64), np.diff(col))
-        layout["_csc_src"] = layout["csc_sources"].astype(np.int64)
-    dst = layout["_csc_dst"]
-    src = layout["_csc_src"]
-    w = layout["csc_weights"]
-    # restrict to edges whose destination tile is selected (contiguous spans)
-    adm = vsel[src] & vsel[dst]
-    cand = (d[src] + w).astype(np.float32)
-    if np.any(adm & (cand < d[dst])):
-        return "an admitted edge still relaxes (not a fixed point)"
-    tight = adm & np.isfinite(d[src]) & (cand == d[dst])
-    has = np.zeros(len(d), bool)
-    has[dst[tight]] = True
-    need = np.isfinite(d) & vsel
-    need[srcs] = False
-    if np.any(need & ~has):
-        return "a finite label has no tight admitted predecessor"
-    if np.any(w[adm] <= 0):
-        return "certificate needs strictly positive admitted weights"
-    return ""
-
-
 def certify_batch(layout, ws, sources, tiles, threads=16):
     """Certifies every query of a completed workspace with the oracle's region
     certificate: labels come off the device one query at a time (the

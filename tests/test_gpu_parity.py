@@ -8,7 +8,7 @@ import pytest
 
 import bfnew_amd as bfn
 import oracle
-from helpers import (bits, certify_batch, certify_sssp, csr_view, fbits, fixture_arrays,
+from helpers import (bits, certify_batch, csr_view, fbits, fixture_arrays,
                      oracle_query, oracle_region, stratified, tile_mask)
 
 pytestmark = pytest.mark.gpu
@@ -349,6 +349,40 @@ def test_push_window(device):
         with pytest.raises(bfn.BfnError) as refused:
             run(device, dg, batch, bfn.FRONTIER_PUSH, bfn.PERSISTENT, 0, push_window=bad)
         assert refused.value.code == 1
+
+
+def test_update_weights_orientations(device):
+    """bfn_graph_update_weights: new weights in both orientations (each edge's
+    weight doubled) give the oracle's distances on the doubled weights for a
+    push and a pull engine; orientations that do not hold the same weights,
+    or arrays of the wrong length, are refused and leave the weights as they
+    were."""
+    hg, dg = synthetic(device, 16, 16, 64, 5)
+    view = hg.view()
+    lay = hg.layout_arrays()
+    qs, batch = c2_batch(hg, 6, seed=21)
+    csr_w = (lay["csr_weights"] * np.float32(2)).astype(np.float32)
+    csc_w = (lay["csc_weights"] * np.float32(2)).astype(np.float32)
+    dg.update_weights(csr_w, csc_w)
+    assert dg.info().has_zero_weight == 0
+    doubled = dict(view, weights=(view["weights"] * np.float32(2)).astype(np.float32))
+    refs = [oracle_query(doubled, [q.source], batch.tiles[i])[0] for i, q in enumerate(qs)]
+    for eng in (bfn.FRONTIER_PUSH, bfn.JACOBI_PULL):
+        ws = run(device, dg, batch, eng, bfn.PERSISTENT, 0)
+        for i in range(len(qs)):
+            assert np.array_equal(bits(ws.distances(i)), bits(refs[i])), (eng, i)
+        ws.close()
+    bad = csc_w.copy()
+    bad[0] = np.float32(bad[0] + 1)
+    with pytest.raises(bfn.BfnError) as refused:
+        dg.update_weights(csr_w, bad)
+    assert refused.value.code == 1
+    with pytest.raises(ValueError):
+        dg.update_weights(csr_w[:-1], csc_w[:-1])
+    ws = run(device, dg, batch, bfn.FRONTIER_PUSH, bfn.PERSISTENT, 0)
+    for i in range(len(qs)):
+        assert np.array_equal(bits(ws.distances(i)), bits(refs[i])), i
+    ws.close()
 
 
 def test_relaxation_counts_match_oracle(device):
