@@ -467,6 +467,11 @@ def main():
         "cycles_per_visit": sum(cycles) / max(1, counters.tile_visits),
         "atomic_attempts": counters.atomic_attempts,
         "atomic_successes": counters.atomic_successes,
+        # work per work-normalised relaxation: edge records the visits load,
+        # and candidate labels they compute (internal edges from frontier
+        # sources per closure sweep, plus boundary candidates)
+        "records_per_relaxation": counters.edges_relaxed / max(1, relax_per_step),
+        "candidates_per_relaxation": counters.candidate_evaluations / max(1, relax_per_step),
     }
     log("visit profile (instrumented run):", json.dumps(visit_profile))
 

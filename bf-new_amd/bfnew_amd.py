@@ -163,7 +163,7 @@ class EngineCounters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("tile_visits", "edges_relaxed", "algorithmic_bytes",
                                           "atomic_attempts", "atomic_successes", "maximum_list",
                                           "rounds_executed", "closure_sweeps")] + [
-        ("phase_cycles", C.c_uint64 * len(PHASE_NAMES))]
+        ("phase_cycles", C.c_uint64 * len(PHASE_NAMES)), ("candidate_evaluations", C.c_uint64)]
 
 
 VERTEX_DTYPE = np.dtype([("x", "<i4"), ("y", "<i4"), ("has_location", "<u4"),

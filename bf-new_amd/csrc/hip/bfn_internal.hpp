@@ -139,6 +139,7 @@ struct DevControl {
   unsigned long long edges_relaxed;      // edge records evaluated by visits
   unsigned long long algorithmic_bytes;  // modelled HBM bytes of the visits
   unsigned long long closure_sweeps;     // tile-closure sweeps (one per block pass)
+  unsigned long long candidates;         // candidate labels computed by the visits
   unsigned long long phase_cycles[kPhaseCount];  // summed over workgroups
   // Slot-order claims: the compacted list of the round is cut into one
   // segment per XCD group, [seg_begin[x], seg_begin[x+1]) (read once per

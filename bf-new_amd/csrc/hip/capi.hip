@@ -1377,6 +1377,7 @@ int32_t bfn_batch_counters(bfn_workspace ws, bfn_engine_counters* out) {
   out->rounds_executed = c->rounds_executed;
   out->closure_sweeps = c->closure_sweeps;
   for (int p = 0; p < kPhaseCount; ++p) out->phase_cycles[p] = c->phase_cycles[p];
+  out->candidate_evaluations = c->candidates;
   return BFN_OK;
 }
 

@@ -114,7 +114,8 @@ constexpr int stream_records(int block) { return block >= 512 ? 8 : 4; }
 // 3 boundary records, 4 write-back and enqueue.
 template <bool I>
 struct VisitStats {
-  unsigned long long attempts = 0, successes = 0, bytes = 0, edges = 0, items = 0, sweeps = 0;
+  unsigned long long attempts = 0, successes = 0, bytes = 0, edges = 0, items = 0, sweeps = 0,
+                     evals = 0;
   unsigned long long phase[kPhaseCount] = {};
   long long t = 0;
   __device__ __forceinline__ void start() {
@@ -188,6 +189,7 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
   }
   // Pass 1: every admitted in-edge reads the staged read column.
   auto relax_staged = [&](std::uint32_t p, float w) {
+    if constexpr (I) ++st.evals;
     const std::uint32_t cb = as_bits(as_float(l.work[l.slots[pk_k(p)] + pk_src(p)]) + w);
     if (cb < l.own[pk_dst(p)]) atomicMin(&l.own[pk_dst(p)], cb);
   };
@@ -229,6 +231,7 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
         for (int j = 0; j < EPT; ++j) {
           const std::uint32_t p = internal.pack[j];
           if (p == kNone) continue;
+          if constexpr (I) ++st.evals;
           const std::uint32_t cb = as_bits(as_float(l.own[pk_src(p)]) + internal.w[j]);
           if (cb < l.own[pk_dst(p)]) {
             atomicMin(&l.own[pk_dst(p)], cb);
@@ -350,6 +353,7 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
         if (p == kNone) continue;
         const std::uint32_t sv = l.own[pk_src(p)];
         if (!(sv < l.work[pk_src(p)])) continue;
+        if constexpr (I) ++st.evals;
         const std::uint32_t cb = as_bits(as_float(sv) + internal.w[j]);
         if (cb < l.own[pk_dst(p)]) {
           atomicMin(&l.own[pk_dst(p)], cb);
@@ -468,7 +472,8 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
           if (cb < seen[c]) {
             atomic_min_agent(target(pk[c]), cb);
             ++suc;
-            atomicMin(&l.marks[pk_k(pk[c])], cb);
+            // read first: most candidates into one neighbour lose to its mark
+            if (cb < l.marks[pk_k(pk[c])]) atomicMin(&l.marks[pk_k(pk[c])], cb);
           }
         }
       }
@@ -480,6 +485,7 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
   }
   st.attempts += att;
   st.successes += suc;
+  st.evals += att;
   // direct: a remote label read per attempt; both: read-modify-write per
   // atomic issued (proposal slabs read once, counted above)
   st.bytes += (proposals ? 0ull : 4ull * att) + 8ull * suc;
@@ -638,6 +644,7 @@ __device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch&
     if (st.successes) atomicAdd(&c->atomic_successes, st.successes);
     if (st.bytes) atomicAdd(&c->algorithmic_bytes, st.bytes);
     if (st.edges) atomicAdd(&c->edges_relaxed, st.edges);
+    if (st.evals) atomicAdd(&c->candidates, st.evals);
     if (threadIdx.x == 0) {
       if (st.items) atomicAdd(&c->items_processed, st.items);
       if (st.sweeps) atomicAdd(&c->closure_sweeps, st.sweeps);

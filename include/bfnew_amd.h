@@ -334,6 +334,10 @@ typedef struct {
                                     over workgroups: setup and label staging,
                                     internal records, tile closure, boundary
                                     records, write-back and enqueue */
+  uint64_t candidate_evaluations;  /* candidate labels fl(d[u] + w) computed:
+                                    internal edges from a frontier source per
+                                    closure sweep (every edge per pull pass),
+                                    plus boundary candidates */
 } bfn_engine_counters;
 int32_t bfn_batch_counters(bfn_workspace workspace, bfn_engine_counters *out);
 
