@@ -196,9 +196,17 @@ def plain_vs_batched(dev, steps):
     hg = bfn.HostGraph.synthetic(*C2["tiles"], C2["seed"])
     dg = bfn.DeviceGraph(dev, hg.layout())
     qs, batch = make_batch(hg, C2)
+    # The batched run takes the autotuner's choice for this workload (block
+    # Jacobi in one persistent launch when there is none).
+    tune_entry = load_tuning(C2["name"])
+    tune = tuned_defaults(tune_entry or dict(sweeps=0))
     ws = bfn.Workspace(dev)
-    ws.submit(dg, batch, bfn.run_options(engine=bfn.JACOBI_PULL, control_mode=bfn.PERSISTENT),
-              bfn.tuning(tile_sweeps=0, count_relaxations=1))
+    ws.submit(dg, batch, bfn.run_options(engine=tune["engine"], control_mode=tune["control"],
+                                         block_size=tune["block"]),
+              bfn.tuning(tile_sweeps=tune["sweeps"], count_relaxations=1,
+                         items_per_claim=tune["items_per_claim"], use_graph=tune["use_graph"],
+                         graph_rounds=tune["graph_rounds"], boundary_mode=tune["boundary_mode"],
+                         min_waves=tune["min_waves"], **scheduling(tune)))
     _, per, _ = ws.status()
     relax = [p.relaxations for p in per]
     elapsed, _, _, _ = timed_steps(ws, steps, 1)
@@ -225,8 +233,10 @@ def plain_vs_batched(dev, steps):
         for w in wss:
             w.close()
     best_plain = max(v["value"] for v in out.values())
-    return dict(workload=C2["name"], batched_value=batched, plain=out,
-                batched_over_best_plain=batched / best_plain)
+    return dict(workload=C2["name"], batched_value=batched,
+                batched_engine=bfn.ENGINE_NAMES[tune["engine"]],
+                batched_control=bfn.CONTROL_NAMES[tune["control"]] + ("+hipgraph" if tune["use_graph"] else ""),
+                plain=out, batched_over_best_plain=batched / best_plain)
 
 
 SERVING = dict(name="c4_serving_64x64x256", tiles=(64, 64, 256), seed=2, query_seed=44,
