@@ -2,12 +2,13 @@
 // persistent relaxation kernel's control loop (the reference measures
 // cooperative_groups grid.sync the same way, tools/gpu_barrier_benchmark.hip.cpp).
 //
-// For workgroups of 256 threads at 1, 2, 4 and the occupancy limit per CU,
-// one cooperative launch runs `rounds` barriers with a little arithmetic
+// For workgroups of 256 threads at 1, 2, 4, 5 and the occupancy limit per
+// CU, one cooperative launch runs `rounds` barriers with a little arithmetic
 // between them, timed with HIP events; reported per barrier:
-//   flat     -- every workgroup arrives on one counter (grid_barrier_flat)
-//   grouped  -- arrivals gathered per XCD group first (grid_barrier_grouped)
-//   cg       -- cooperative_groups::this_grid().sync()
+//   flat        -- every workgroup arrives on one counter (grid_barrier_flat)
+//   grouped_G   -- arrivals gathered on G group counters first
+//                  (grid_barrier_grouped<G>, G = 8, 16, 32, 64)
+//   cg          -- cooperative_groups::this_grid().sync()
 // One JSON object per line on stdout.
 #include "bfn_internal.hpp"
 
@@ -29,10 +30,11 @@ __global__ __launch_bounds__(256) void barrier_kernel(DevControl* c, std::uint32
     acc = acc * 6364136223846793005ull + r + threadIdx.x;
     if constexpr (Kind == 0) {
       if (!bfn::grid_barrier_flat(c, gridDim.x)) break;
-    } else if constexpr (Kind == 1) {
-      if (!bfn::grid_barrier_grouped(c, gridDim.x)) break;
+    } else if constexpr (Kind > 0) {
+      if (!bfn::grid_barrier_grouped<static_cast<std::uint32_t>(Kind)>(c, gridDim.x)) break;
+    } else {
+      cooperative_groups::this_grid().sync();
     }
-    else cooperative_groups::this_grid().sync();
   }
   if (threadIdx.x == 0) sink[blockIdx.x] = acc;
 }
@@ -83,17 +85,22 @@ int main(int argc, char** argv) {
   const std::uint32_t cus = static_cast<std::uint32_t>(p.multiProcessorCount);
   CHECK_HIP(hipMalloc(&c, sizeof(DevControl)));
   CHECK_HIP(hipMalloc(&sink, sizeof(unsigned long long) * cus * static_cast<std::uint32_t>(occ)));
-  std::vector<std::uint32_t> per_cu{1, 2, 4};
-  if (occ > 4) per_cu.push_back(static_cast<std::uint32_t>(occ));
+  std::vector<std::uint32_t> per_cu{1, 2, 4, 5};
+  if (occ > 5) per_cu.push_back(static_cast<std::uint32_t>(occ));
   for (const std::uint32_t k : per_cu) {
     if (k > static_cast<std::uint32_t>(occ)) continue;
     const std::uint32_t grid = k * cus;
-    const float flat = run<0>(c, sink, grid, rounds);
-    const float grouped = run<1>(c, sink, grid, rounds);
-    const float cg = run<2>(c, sink, grid, rounds);
+    const float us = 1e3f / static_cast<float>(rounds);
+    const float flat = run<0>(c, sink, grid, rounds) * us;
+    const float g8 = run<8>(c, sink, grid, rounds) * us;
+    const float g16 = run<16>(c, sink, grid, rounds) * us;
+    const float g32 = run<32>(c, sink, grid, rounds) * us;
+    const float g64 = run<64>(c, sink, grid, rounds) * us;
+    const float cg = run<-1>(c, sink, grid, rounds) * us;
     std::printf("{\"workgroups\": %u, \"per_cu\": %u, \"rounds\": %u, \"us_per_barrier\": "
-                "{\"flat\": %.3f, \"grouped\": %.3f, \"cg\": %.3f}}\n",
-                grid, k, rounds, flat * 1e3f / rounds, grouped * 1e3f / rounds, cg * 1e3f / rounds);
+                "{\"flat\": %.3f, \"grouped_8\": %.3f, \"grouped_16\": %.3f, \"grouped_32\": %.3f, "
+                "\"grouped_64\": %.3f, \"cg\": %.3f}}\n",
+                grid, k, rounds, flat, g8, g16, g32, g64, cg);
     std::fflush(stdout);
   }
   CHECK_HIP(hipFree(sink));

@@ -62,10 +62,13 @@ constexpr std::uint32_t kLdsFlagWords = 16;
 constexpr std::uint32_t kEnginesMixed = 0;
 constexpr std::uint32_t kEnginesPush = 1;
 constexpr std::uint32_t kEnginesPull = 2;
-// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8), so
-// a barrier that first gathers each XCD's workgroups spreads its arrivals
-// over 8 counters instead of one.
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8):
+// per-XCD counters (claims, window minima) are indexed by that group.  The
+// grid barrier gathers its arrivals on kArrivalGroups counters (a multiple
+// of 8, so each counter's workgroups share an XCD) before the top counter.
 constexpr std::uint32_t kBarrierGroups = 8;
+constexpr std::uint32_t kMaxArrivalGroups = 64;  // arrival counters in DevControl
+constexpr std::uint32_t kArrivalGroups = 32;     // used by the relaxation kernels
 constexpr std::uint32_t kBarrierStride = 32;  // words: one 128-byte line per group
 
 enum Engine : std::uint32_t { kJacobiPull = 0, kDensePush = 1, kFrontierPush = 2 };
@@ -120,18 +123,22 @@ struct DevGraph {
 };
 
 // Control block; lives in device memory, one per workspace.
+// Words that workgroups hit with atomics or polls during a round sit on
+// 128-byte lines of their own: agent-scope atomics and coherent loads of one
+// line are served one after another at the memory side, so enqueues
+// (list_count), claims (claim) and the barrier polls of finished workgroups
+// (barrier_generation) would otherwise queue behind each other.
 struct DevControl {
-  std::uint32_t list_count[3];
-  std::uint32_t claim[3];
   std::uint32_t round;            // next round to execute
   std::uint32_t done;
   std::uint32_t stop_reason;
   std::uint32_t error_bits;
-  std::uint32_t barrier_count;
-  std::uint32_t barrier_generation;
   std::uint32_t rounds_executed;
-  std::uint32_t pad[3];
-  unsigned long long items_processed;
+  alignas(128) std::uint32_t list_count[3];
+  alignas(128) std::uint32_t claim[3];
+  alignas(128) std::uint32_t barrier_count;
+  alignas(128) std::uint32_t barrier_generation;
+  alignas(128) unsigned long long items_processed;
   unsigned long long edges_examined;
   unsigned long long atomic_attempts;
   unsigned long long atomic_successes;
@@ -154,9 +161,9 @@ struct DevControl {
   // line); the limit of round r + 1 is that minimum plus the window.
   std::uint32_t window_limit;
   alignas(128) std::uint32_t next_min[3 * kBarrierGroups * kBarrierStride];
-  // Arrival counters of the two-level grid barrier, one per XCD group of
-  // workgroups (blockIdx % 8), each on its own 128-byte line.
-  alignas(128) std::uint32_t group_arrivals[kBarrierGroups * kBarrierStride];
+  // Arrival counters of the two-level grid barrier, one per group of
+  // workgroups (blockIdx % G), each on its own 128-byte line.
+  alignas(128) std::uint32_t group_arrivals[kMaxArrivalGroups * kBarrierStride];
 };
 
 // What a visit of slot s needs before it can load any label: written once
@@ -389,21 +396,27 @@ __device__ __forceinline__ bool grid_barrier_flat(DevControl* c, std::uint32_t b
 }
 
 // Two-level: a workgroup arrives on its group's counter (group =
-// blockIdx % 8, the XCD it was dispatched to); the last arriver of a group
-// resets that counter and arrives on the global one; the last group to
-// arrive resets the global counter and advances the generation.  Every
+// blockIdx % G; G is a multiple of 8, so every group's workgroups sit on the
+// XCD that the dispatcher's round-robin placed them on); the last arriver of
+// a group resets that counter and arrives on the global one; the last group
+// to arrive resets the global counter and advances the generation.  Every
 // counter is reset before the generation moves, so the next barrier's
-// arrivals always find zeros.
+// arrivals always find zeros.  On MI355X one barrier of 1280 workgroups
+// costs 21.4 / 20.6 / 20.4 / 20.6 us with G = 8 / 16 / 32 / 64 (31.7 us
+// with one counter): the arrivals are not the part that grows with the
+// grid, and G = 32 is the measured best.
+template <std::uint32_t G = kArrivalGroups>
 __device__ __forceinline__ bool grid_barrier_grouped(DevControl* c, std::uint32_t blocks,
                                                      std::uint32_t limit = kBarrierSpinLimit) {
+  static_assert(G % kBarrierGroups == 0 && G <= kMaxArrivalGroups, "arrival groups");
   __syncthreads();
   bool ok = true;
   if (threadIdx.x == 0) {
     const std::uint32_t gen = load_agent(&c->barrier_generation);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    const std::uint32_t g = blockIdx.x % kBarrierGroups;
-    const std::uint32_t members = (blocks + kBarrierGroups - 1u - g) / kBarrierGroups;
-    const std::uint32_t groups = blocks < kBarrierGroups ? blocks : kBarrierGroups;
+    const std::uint32_t g = blockIdx.x % G;
+    const std::uint32_t members = (blocks + G - 1u - g) / G;
+    const std::uint32_t groups = blocks < G ? blocks : G;
     std::uint32_t* mine = &c->group_arrivals[g * kBarrierStride];
     bool last = false;
     if (atomic_add_agent(mine, 1u) == members - 1) {

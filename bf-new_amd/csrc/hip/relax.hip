@@ -161,7 +161,7 @@ __global__ void reset_control_kernel(DevControl* c, std::uint32_t window) {
     c->closure_sweeps = 0;
     c->candidates = 0;
     for (int p = 0; p < kPhaseCount; ++p) c->phase_cycles[p] = 0;
-    for (std::uint32_t i = 0; i < kBarrierGroups * kBarrierStride; ++i) c->group_arrivals[i] = 0;
+    for (std::uint32_t i = 0; i < kMaxArrivalGroups * kBarrierStride; ++i) c->group_arrivals[i] = 0;
   }
 }
 
