@@ -132,6 +132,29 @@ def timed_steps(ws, steps, warmup, paths=False):
     return time.perf_counter() - t0, relax_ms, ws.timing()[2], paths_ms
 
 
+def persistent_control(dev, dg, batch, tune, target_bits, steps):
+    """The same batch and tuning under one persistent cooperative launch per
+    step (grid barrier between rounds) instead of the tuned control: wall
+    and relaxation-kernel ms per step, and whether every target distance
+    has the bits of the tuned run."""
+    ws = bfn.Workspace(dev)
+    try:
+        ws.submit(dg, batch, bfn.run_options(engine=tune["engine"], control_mode=bfn.PERSISTENT,
+                                             block_size=tune["block"]),
+                  bfn.tuning(tile_sweeps=tune["sweeps"], items_per_claim=tune["items_per_claim"],
+                             boundary_mode=tune["boundary_mode"], min_waves=tune["min_waves"],
+                             push_window=tune.get("push_window", 0.0)))
+        st, _, _ = ws.status()
+        bits = np.array([p["distance"] for p in ws.paths()], np.float32).view(np.uint32)
+        elapsed, relax_ms, _, _ = timed_steps(ws, steps, 1)
+        return {"control": "persistent_cooperative", "steps": steps,
+                "ms_per_step": elapsed * 1e3 / steps, "kernel_ms_per_step": float(np.mean(relax_ms)),
+                "rounds": int(ws.counters().rounds_executed), "converged": bool(st.converged),
+                "same_target_bits": bool(np.array_equal(bits, target_bits))}
+    finally:
+        ws.close()
+
+
 def cpu_baseline(hg_view, qs, tiles, budget_s):
     """The reference CPU path (dijkstra_oracle built from the reference's own
     sources, oracle/_ref) on a bounded sample of the same queries: one core,
@@ -232,6 +255,8 @@ def plain_vs_batched(dev, steps):
             p99_ms=percentile(lat, 99) * 1e3, control="persistent_cooperative, one launch per query")
         for w in wss:
             w.close()
+    ws.close()
+    dg.close()
     best_plain = max(v["value"] for v in out.values())
     return dict(workload=C2["name"], batched_value=batched,
                 batched_engine=bfn.ENGINE_NAMES[tune["engine"]],
@@ -468,6 +493,9 @@ def main():
     log(f"{value / 1e9:.2f} G relax/s, {ms_per_step:.2f} ms/step, relax kernel {kernel_ms:.3f} ms, "
         f"compact paths {float(np.mean(paths_ms)):.3f} ms, "
         f"{counters.tile_visits} tile visits, {counters.rounds_executed} rounds")
+    target_bits = np.array([p["distance"] for p in ws.paths()], np.float32).view(np.uint32)
+    persistent = persistent_control(dev, dg, batch, tune, target_bits, max(3, args.steps // 2))
+    log("persistent cooperative control:", json.dumps(persistent))
     traffic, traffic_source = profiled_traffic(C3["name"], tune)
     cycles = [int(c) for c in counters.phase_cycles]
     visit_profile = {
@@ -523,6 +551,7 @@ def main():
                      "algorithmic_bytes_per_step": counters.algorithmic_bytes,
                      "edges_relaxed_per_step": counters.edges_relaxed},
         "visit_profile": visit_profile,
+        "persistent_control": persistent,
     }
     if not args.no_plain:
         pm = plain_vs_batched(dev, max(3, args.steps // 2))
@@ -546,6 +575,11 @@ def main():
         result["cpu_baseline"] = cb
         result["speedup_vs_cpu"] = value / cb["value"]
         log("cpu baseline:", json.dumps(cb))
+    # Every handle is released before the interpreter exits: a workspace left
+    # to process teardown still holds its stream and hipGraph executable.
+    ws.close()
+    dg.close()
+    dev.close()
     print(json.dumps(result), flush=True)
 
 
