@@ -68,7 +68,7 @@ def profiled_traffic(workload, tune):
         return None, None
     cfg = entry["config"]
     keys = ("engine", "block", "sweeps", "boundary_mode", "use_graph", "min_waves", "claim_order",
-            "push_window")
+            "push_window", "items_per_claim")
     if any(float(cfg.get(k, 0)) != float(tune.get(k, 0)) for k in keys):
         return None, None
     return entry["hbm_bytes_per_step"], entry["source"]
