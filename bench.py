@@ -111,13 +111,14 @@ def apply_tuning(batch, qs, tune_entry):
 def timed_steps(ws, steps, warmup, paths=False):
     """Times `steps` reruns of the resident batch (with paths: each followed
     by compact-path reconstruction of every target and the copy of the
-    summaries and path arenas to the host).  Returns the wall time, the
-    relaxation time of each step (HIP events on the workspace stream), the
-    round launches of the last step and the wall time of each paths call."""
+    summaries, offsets and path arenas to host arrays).  Returns the wall
+    time, the relaxation time of each step (HIP events on the workspace
+    stream), the round launches of the last step and the wall time of each
+    paths call."""
     for _ in range(warmup):
         ws.rerun()
         if paths:
-            ws.paths()
+            ws.path_arrays()
     bfn.hip_lib().bfn_device_synchronize(ws.device.h)
     relax_ms, paths_ms = [], []
     t0 = time.perf_counter()
@@ -126,7 +127,7 @@ def timed_steps(ws, steps, warmup, paths=False):
         relax_ms.append(ws.timing()[0])
         if paths:
             t = time.perf_counter()
-            ws.paths()
+            ws.path_arrays()
             paths_ms.append((time.perf_counter() - t) * 1e3)
     bfn.hip_lib().bfn_device_synchronize(ws.device.h)
     return time.perf_counter() - t0, relax_ms, ws.timing()[2], paths_ms
@@ -537,7 +538,8 @@ def main():
                    (f"+hipgraph({tune['graph_rounds']} rounds)" if tune["use_graph"] else ""),
                    "relaxations_per_step": relax_per_step,
                    "step": "label initialisation + relaxation to the fixed point + per-query "
-                           "finalisation + compact paths of every target copied to the host",
+                           "finalisation + compact paths of every target (summaries, offsets, "
+                           "vertex / label / edge-id arenas) copied to host arrays",
                    "compact_paths_ms_per_step": float(np.mean(paths_ms))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

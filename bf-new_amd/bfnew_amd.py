@@ -638,7 +638,11 @@ class Workspace:
         _check(hip_lib().bfn_batch_predecessors(self.h, query, _ptr(out)))
         return out
 
-    def paths(self):
+    def path_arrays(self) -> dict:
+        """Compact paths of every target as bfn_batch_paths returns them: the
+        target summaries (a record array with the bfn_target_summary fields),
+        the path offsets (target i's path is [offsets[i], offsets[i + 1]) of
+        the arenas) and the vertex, label and edge-id arenas."""
         NT = int(self.batch.target_offsets[-1])
         summ = (TargetSummary * max(NT, 1))()
         offs = np.zeros(NT + 1, np.uint64)
@@ -646,21 +650,28 @@ class Workspace:
         _check(hip_lib().bfn_batch_paths(self.h, summ, _ptr(offs), None, None, None, 0,
                                          C.byref(total)))
         n = max(total.value, 1)
-        verts = np.zeros(n, np.uint32)
-        labels = np.zeros(n, np.float32)
-        edges = np.zeros(n, np.uint32)
+        verts = np.empty(n, np.uint32)
+        labels = np.empty(n, np.float32)
+        edges = np.empty(n, np.uint32)
         _check(hip_lib().bfn_batch_paths(self.h, summ, _ptr(offs), _ptr(verts), _ptr(labels),
                                          _ptr(edges), n, C.byref(total)))
+        return dict(summaries=np.ctypeslib.as_array(summ)[:NT], offsets=offs, vertices=verts,
+                    labels=labels, edge_ids=edges)
+
+    def paths(self):
+        """Compact paths of every target, one dict per target."""
+        a = self.path_arrays()
+        summ, offs = a["summaries"], a["offsets"]
         out = []
-        for i in range(NT):
+        for i in range(len(summ)):
             s = summ[i]
-            a, b = int(offs[i]), int(offs[i + 1])
-            out.append(dict(target=s.target, selected_source=s.selected_source,
-                            distance=s.distance, path_length=s.path_length, reached=s.reached,
-                            reconstruction=s.reconstruction,
-                            has_selected_source=s.has_selected_source,
-                            vertices=verts[a:b].copy(), labels=labels[a:b].copy(),
-                            edge_ids=edges[a:a + max(0, b - a - 1)].copy()))
+            lo, hi = int(offs[i]), int(offs[i + 1])
+            out.append(dict(target=int(s["target"]), selected_source=int(s["selected_source"]),
+                            distance=float(s["distance"]), path_length=int(s["path_length"]),
+                            reached=int(s["reached"]), reconstruction=int(s["reconstruction"]),
+                            has_selected_source=int(s["has_selected_source"]),
+                            vertices=a["vertices"][lo:hi].copy(), labels=a["labels"][lo:hi].copy(),
+                            edge_ids=a["edge_ids"][lo:lo + max(0, hi - lo - 1)].copy()))
         return out
 
     def occupancy(self) -> np.ndarray:

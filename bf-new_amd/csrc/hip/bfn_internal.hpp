@@ -247,7 +247,7 @@ struct PathArgs {
   std::uint32_t jacobi_copy;            // column holding final Jacobi labels
   std::uint32_t pass;                   // 1: summaries + lengths, 2: emit arenas
   std::uint32_t target_total;
-  std::uint32_t pad;
+  std::uint32_t stamp_base;             // target t's walk marks onpath with stamp_base + t + 1
   const std::uint32_t* query_clean;     // [Q] converged without error
   const unsigned long long* frame_base; // [Q] frame scratch offset
   std::uint32_t* frame_vertex;

@@ -55,13 +55,12 @@ struct DeviceBuffer {
   // Geometric growth, like DeviceBuffer::reserve (runtime.hip.cpp:222-240).
   hipError_t reserve(std::size_t bytes) {
     if (bytes <= capacity && ptr) return hipSuccess;
+    const std::size_t grown = std::max({bytes, std::size_t{256}, capacity + capacity / 2});
     if (ptr) {
       (void)hipFree(ptr);
       ptr = nullptr;
       capacity = 0;
     }
-    std::size_t grown = std::max<std::size_t>(bytes, 256);
-    grown = std::max(grown, capacity + capacity / 2);
     hipError_t e = hipMalloc(&ptr, grown);
     if (e == hipSuccess) capacity = grown;
     return e;
@@ -181,6 +180,12 @@ struct bfn_workspace_s {
   bool submitted = false, completed = false, have_paths = false;
   bool paths_counted = false;  // pass 1 of this completion's paths is on the host
   std::uint64_t arena_vertices = 0;
+  // Path-kernel marks in `onpath`: the next pass marks with stamps
+  // [path_stamp + 1, path_stamp + targets]; the buffer was last cleared at
+  // onpath_cleared for onpath_cleared_words words.
+  std::uint32_t path_stamp = 0;
+  const void* onpath_cleared = nullptr;
+  std::uint64_t onpath_cleared_words = 0;
   std::uint64_t launches = 0, host_checks = 0, host_syncs = 0, controller_copies = 0;
   hipEvent_t ev_start = nullptr, ev_relax0 = nullptr, ev_relax1 = nullptr, ev_stop = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -1308,6 +1313,24 @@ int32_t bfn_batch_paths(bfn_workspace ws, bfn_target_summary* summaries, uint64_
   a.summaries = ws->paths_summary.as<std::uint32_t>();
   a.lengths = ws->paths_len.as<unsigned long long>();
   a.offsets = ws->paths_off.as<unsigned long long>();
+  // Every pass of the path kernel marks the vertices on its walks with its
+  // own stamp range, so the marks that earlier passes (and earlier batches)
+  // left on found paths never match a later stamp: the marks are cleared
+  // only when the buffer is new or larger than what was cleared, or when the
+  // 32-bit stamps would wrap.
+  auto take_stamps = [&]() -> hipError_t {
+    if (ws->onpath_cleared != ws->onpath.ptr || ws->onpath_cleared_words < ws->pool_words ||
+        ws->path_stamp > 0xFFFFFFFFu - NT) {
+      const hipError_t e = fill_launch(ws->onpath.as<std::uint32_t>(), ws->pool_words, 0u, s);
+      if (e != hipSuccess) return e;
+      ws->onpath_cleared = ws->onpath.ptr;
+      ws->onpath_cleared_words = ws->pool_words;
+      ws->path_stamp = 0;
+    }
+    a.stamp_base = ws->path_stamp;
+    ws->path_stamp += NT;
+    return hipSuccess;
+  };
   // Pass 1 (summaries, lengths, offsets) runs once per completed batch: a
   // second call, the one that brings the arenas, reuses its results.
   if (!ws->paths_counted) {
@@ -1315,7 +1338,7 @@ int32_t bfn_batch_paths(bfn_workspace ws, bfn_target_summary* summaries, uint64_
     for (std::uint32_t q = 0; q < ws->Q; ++q)
       hclean[q] = (qs[q].converged && qs[q].error_bits == 0) ? 1u : 0u;
     BFN_HIP(hipMemcpyAsync(ws->clean.ptr, hclean, 4ull * ws->Q, hipMemcpyHostToDevice, s));
-    BFN_HIP(fill_launch(ws->onpath.as<std::uint32_t>(), ws->pool_words, 0u, s));
+    BFN_HIP(take_stamps());
     BFN_HIP(paths_launch(a, ws->Q, s));
     BFN_HIP(scan_u64_launch(a.lengths, ws->paths_off.as<unsigned long long>(), NT, s));
     BFN_HIP(hipMemcpyAsync(hp, ws->paths_summary.ptr, 28ull * NT, hipMemcpyDeviceToHost, s));
@@ -1333,7 +1356,7 @@ int32_t bfn_batch_paths(bfn_workspace ws, bfn_target_summary* summaries, uint64_
   BFN_HIP(ws->arena_v.reserve(4ull * std::max<std::uint64_t>(total, 1)));
   BFN_HIP(ws->arena_l.reserve(4ull * std::max<std::uint64_t>(total, 1)));
   BFN_HIP(ws->arena_e.reserve(4ull * std::max<std::uint64_t>(total, 1)));
-  BFN_HIP(fill_launch(ws->onpath.as<std::uint32_t>(), ws->pool_words, 0u, s));
+  BFN_HIP(take_stamps());
   a.pass = 2;
   a.arena_vertices = ws->arena_v.as<std::uint32_t>();
   a.arena_labels = ws->arena_l.as<std::uint32_t>();

@@ -81,7 +81,7 @@ __global__ __launch_bounds__(64) void compact_paths_kernel(PathArgs a) {
       }
       continue;
     }
-    const std::uint32_t stamp = (a.pass - 1) * a.target_total + t + 1;
+    const std::uint32_t stamp = a.stamp_base + t + 1;
     std::int32_t top = 0;
     std::uint32_t v = target;
     std::uint32_t last = kNone;  // EdgeId last tried from the top frame

@@ -358,6 +358,7 @@ hipError_t bfn_server_s::step(std::uint32_t admit_count, std::uint32_t tile_coun
     a.jacobi_copy = 0;  // resolved per query below through the engine column
     a.pass = 1;
     a.target_total = Qcap;
+    a.stamp_base = 0;  // pass 2 marks with [Qcap + 1, 2 Qcap]
     a.query_clean = d_clean;
     a.query_list = d_done + 1;
     a.frame_base = d_frame_base;
@@ -388,6 +389,7 @@ hipError_t bfn_server_s::step(std::uint32_t admit_count, std::uint32_t tile_coun
                               s)) != hipSuccess)
         return e;
       a.pass = 2;
+      a.stamp_base = Qcap;
       a.offsets = d_path_offsets;
       a.arena_vertices = d_arena_v;
       a.arena_labels = d_arena_l;

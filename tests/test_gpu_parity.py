@@ -168,6 +168,48 @@ def test_compact_path_fixture(device, golden):
         ws.close()
 
 
+def same_path_arrays(a, b):
+    return (a["summaries"].tobytes() == b["summaries"].tobytes() and
+            all(np.array_equal(a[k], b[k]) for k in ("offsets", "vertices", "labels", "edge_ids")))
+
+
+def test_compact_paths_repeat_and_rerun(device, golden):
+    """The path kernel's marks stay in the workspace between passes (each
+    pass marks with its own stamps): repeated path calls on one completed
+    batch, and the calls after a rerun, return the same summaries and
+    arenas, and the paths still match the reference's literal paths."""
+    cc = golden["compact"]
+    hg, dg = fixture_graph(device, cc)
+    view = hg.view()
+    qs = cc["queries"]
+    o2n = view["old_to_new"]
+    batch = bfn.Batch(sources=[[int(o2n[s]) for s in q["sources"]] for q in qs],
+                      targets=[[int(o2n[t]) for t in q["targets"]] for q in qs],
+                      tiles=[q["selected_tiles"] for q in qs])
+    hs, ds = synthetic(device, 32, 32, 128, 6)
+    sq, sbatch = c2_batch(hs, 8, seed=7)
+    for graph, b in ((dg, batch), (ds, sbatch)):
+        for eng in ENGINES:
+            ws = run(device, graph, b, eng, bfn.PER_ROUND, 0)
+            first = ws.path_arrays()
+            assert same_path_arrays(ws.path_arrays(), first), eng
+            for _ in range(2):
+                ws.rerun()
+                assert same_path_arrays(ws.path_arrays(), first), eng
+                assert same_path_arrays(ws.path_arrays(), first), eng
+            ws.close()
+    ws = run(device, dg, batch, bfn.FRONTIER_PUSH, bfn.PER_ROUND)
+    for _ in range(3):
+        ws.path_arrays()
+    paths = ws.paths()
+    for qi, q in enumerate(qs):
+        for p in paths[batch.target_offsets[qi]:batch.target_offsets[qi + 1]]:
+            lit = q["expected_paths"].get(str(int(view["new_to_old"][p["target"]])))
+            if lit is not None:
+                assert [int(view["new_to_old"][x]) for x in p["vertices"]] == lit["vertices"]
+    ws.close()
+
+
 def synthetic(device, tx, ty, npt, seed, host_graph=True):
     hg = bfn.HostGraph.synthetic(tx, ty, npt, seed, host_graph=host_graph)
     return hg, bfn.DeviceGraph(device, hg.layout())
