@@ -57,8 +57,11 @@ BFN_RELAX_ENGINE_SET(mixed)
 BFN_RELAX_ENGINE_SET(push)
 BFN_RELAX_ENGINE_SET(pull)
 #undef BFN_RELAX_ENGINE_SET
+// Resets the control block, fills the labels and seeds round 0; with
+// `structure` also resolves the slots' neighbour slots and descriptors (the
+// first run of a batch; its reruns keep them).
 hipError_t prep_launch(const DevGraph& g, const DevBatch& b, std::uint64_t pool_words,
-                       hipStream_t stream);
+                       hipStream_t stream, bool structure = true);
 // Workgroups per CU (at most `promised`) whose persistent grid the device
 // actually places at once, checked by a probe launch of the variant that `b`
 // selects (b.engines, b.min_waves, b.instrumentation; b.control is reset and

@@ -179,6 +179,7 @@ struct bfn_workspace_s {
   const unsigned long long* frame_base_ptr = nullptr;  // in `meta`
   bool submitted = false, completed = false, have_paths = false;
   bool paths_counted = false;  // pass 1 of this completion's paths is on the host
+  bool slots_resolved = false; // the batch's neighbour slots and descriptors are on the device
   std::uint64_t arena_vertices = 0;
   // Path-kernel marks in `onpath`: the next pass marks with stamps
   // [path_stamp + 1, path_stamp + targets]; the buffer was last cleared at
@@ -931,7 +932,8 @@ int32_t run_resident_batch(bfn_workspace ws) {
       BFN_HIP(hipGraphInstantiate(&ws->graph_exec, captured, nullptr, nullptr, 0));
       ws->graph_key = key;
     }
-    BFN_HIP(prep_launch(g->view, b, ws->pool_words, s));
+    BFN_HIP(prep_launch(g->view, b, ws->pool_words, s, !ws->slots_resolved));
+    ws->slots_resolved = true;
     BFN_HIP(hipEventRecord(ws->ev_relax0, s));
     BFN_HIP(hipGraphLaunch(ws->graph_exec, s));
     ws->launches += 2ull * rounds;
@@ -959,7 +961,8 @@ int32_t run_resident_batch(bfn_workspace ws) {
     return BFN_OK;
   }
 
-  BFN_HIP(prep_launch(g->view, b, ws->pool_words, s));
+  BFN_HIP(prep_launch(g->view, b, ws->pool_words, s, !ws->slots_resolved));
+  ws->slots_resolved = true;
   BFN_HIP(hipEventRecord(ws->ev_relax0, s));
   if (o.control_mode == 0) {
     BFN_HIP(relax_launch(ws->block, true, true, ws->grid, ws->lds_bytes, ws->max_v,
@@ -1010,6 +1013,7 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
   // refused submit never leaves the previous batch's results readable under
   // the new batch's description.
   ws->submitted = ws->completed = ws->have_paths = ws->paths_counted = false;
+  ws->slots_resolved = false;
   if (ws->device != graph->device) return fail(BFN_E_INVALID_ARGUMENT, "graph is on another device");
   if (int32_t rc = validate_options(o)) return rc;
   bfn_tuning t;

@@ -173,21 +173,28 @@ __global__ void fill_u32_kernel(std::uint32_t* p, unsigned long long n, std::uin
     p[i] = v;
 }
 
-// Resolves each slot's in/out neighbour tiles to the query's slots.
-__global__ void prep_slots_kernel(DevGraph g, DevBatch b) {
+// Per run: each slot's queue stamp, window priority and its query's last
+// change, and the engine-set check.  With `structure` (the first run of a
+// batch) also the slot's in/out neighbour tiles resolved to the query's
+// slots and its descriptor: they depend only on the batch and the graph's
+// tile directory, so the reruns of a batch keep them.
+__global__ void prep_slots_kernel(DevGraph g, DevBatch b, std::uint32_t structure) {
   const std::uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= b.slot_count) return;
   const std::uint32_t q = b.slot_query[s];
-  const std::uint32_t T = b.tiles[s];
-  const std::uint32_t in0 = g.in_nbr_offsets[T], in1 = g.in_nbr_offsets[T + 1];
-  const std::uint32_t ib = b.slot_in_base[s];
-  for (std::uint32_t k = in0; k < in1; ++k) b.slot_in[ib + (k - in0)] = find_slot(b, q, g.in_nbr_tiles[k]);
-  const std::uint32_t o0 = g.out_nbr_offsets[T], o1 = g.out_nbr_offsets[T + 1];
-  const std::uint32_t obs = b.slot_out_base[s];
-  for (std::uint32_t k = o0; k < o1; ++k) {
-    const std::uint32_t ns = find_slot(b, q, g.out_nbr_tiles[k]);
-    b.slot_out[obs + (k - o0)] = ns;
-    if (b.slot_out_off) b.slot_out_off[obs + (k - o0)] = ns == kNone ? ~0ull : b.slot_dist[ns];
+  if (structure) {
+    const std::uint32_t T = b.tiles[s];
+    const std::uint32_t in0 = g.in_nbr_offsets[T], in1 = g.in_nbr_offsets[T + 1];
+    const std::uint32_t ib = b.slot_in_base[s];
+    for (std::uint32_t k = in0; k < in1; ++k)
+      b.slot_in[ib + (k - in0)] = find_slot(b, q, g.in_nbr_tiles[k]);
+    const std::uint32_t o0 = g.out_nbr_offsets[T], o1 = g.out_nbr_offsets[T + 1];
+    const std::uint32_t obs = b.slot_out_base[s];
+    for (std::uint32_t k = o0; k < o1; ++k) {
+      const std::uint32_t ns = find_slot(b, q, g.out_nbr_tiles[k]);
+      b.slot_out[obs + (k - o0)] = ns;
+      if (b.slot_out_off) b.slot_out_off[obs + (k - o0)] = ns == kNone ? ~0ull : b.slot_dist[ns];
+    }
   }
   b.stamp[s] = 0;  // also every queue-bit word of slot-order claims (ceil(S/32) <= S)
   if (b.prio) b.prio[s] = kInfBits;
@@ -196,7 +203,7 @@ __global__ void prep_slots_kernel(DevGraph g, DevBatch b) {
   if ((b.engines == kEnginesPush && pull) || (b.engines == kEnginesPull && !pull))
     atomicOr(&b.control->error_bits, 4u);
   b.last_change[q] = 0;
-  if (b.slot_desc) const_cast<SlotDesc*>(b.slot_desc)[s] = make_slot_desc(g, b, s);
+  if (structure && b.slot_desc) const_cast<SlotDesc*>(b.slot_desc)[s] = make_slot_desc(g, b, s);
 }
 
 // Zero the sources and queue round 0: Jacobi queues the source tiles and
@@ -333,14 +340,14 @@ hipError_t coresident_per_cu(int device, int promised, std::uint32_t cus, std::s
 }
 
 hipError_t prep_launch(const DevGraph& g, const DevBatch& b, std::uint64_t pool_words,
-                       hipStream_t stream) {
+                       hipStream_t stream, bool structure) {
   hipLaunchKernelGGL(reset_control_kernel, dim3(1), dim3(64), 0, stream, b.control, b.window);
   const std::uint32_t fill_grid = 4096;
   hipLaunchKernelGGL(fill_u32_kernel, dim3(fill_grid), dim3(256), 0, stream, b.dist,
                      static_cast<unsigned long long>(pool_words), kInfBits);
   if (b.slot_count)
     hipLaunchKernelGGL(prep_slots_kernel, dim3((b.slot_count + 255) / 256), dim3(256), 0,
-                       stream, g, b);
+                       stream, g, b, structure ? 1u : 0u);
   if (b.query_count)
     hipLaunchKernelGGL(seed_kernel, dim3((b.query_count + 127) / 128), dim3(128), 0, stream, g,
                        b);
