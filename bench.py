@@ -347,6 +347,8 @@ def negotiated(dev, tune):
     for r in range(C5["rounds"]):
         ws.submit(dg, batch, opts, bfn.tuning(tile_sweeps=tune["sweeps"], count_relaxations=1,
                                               items_per_claim=tune["items_per_claim"],
+                                              use_graph=tune["use_graph"],
+                                              graph_rounds=tune["graph_rounds"],
                                               boundary_mode=tune["boundary_mode"],
                                               min_waves=tune["min_waves"], **scheduling(tune)))
         st, per, _ = ws.status()
@@ -365,7 +367,10 @@ def negotiated(dev, tune):
         log(f"negotiated round {r}:", json.dumps(rounds[-1]))
     ws.close()
     dg.close()
-    return {"workload": C5["name"], "engine": bfn.ENGINE_NAMES[tune["engine"]], "rounds": rounds}
+    return {"workload": C5["name"], "engine": bfn.ENGINE_NAMES[tune["engine"]],
+            "control": bfn.CONTROL_NAMES[tune["control"]] +
+            (f"+hipgraph({tune['graph_rounds']} rounds)" if tune["use_graph"] else ""),
+            "tuning": "the headline workload's autotuned entry", "rounds": rounds}
 
 
 def copy_calibration():
@@ -564,7 +569,7 @@ def main():
         result["serving"] = sv
         result["p99_query_latency_ms"] = {r: sv[r]["p99_ms"] for r in map(str, SERVING["rates"])}
     if args.negotiated:
-        result["negotiated"] = negotiated(dev, dict(tune, control=bfn.PERSISTENT))
+        result["negotiated"] = negotiated(dev, tune)
     if expansion_result is not None:
         result["expansion"] = expansion_result
     if not args.no_cpu_baseline:
