@@ -74,8 +74,11 @@ __device__ __forceinline__ Lds carve(std::uint32_t* base, std::uint32_t max_v,
 }
 
 // Boundary records per pushing vertex loaded together (one load, then one
-// filter read, in flight per record).
-constexpr int kPushChunk = 8;
+// filter read, in flight per record).  Six covers a vertex of the tiled
+// routing graphs (four intra-tile and six inter-tile edges per node) in one
+// chunk, and at six the 256-thread push-only kernel fits 79 VGPRs: six
+// workgroups per CU, where eight records per chunk needed 85 (five).
+constexpr int kPushChunk = 6;
 
 // Edge records in registers: thread t holds records t, t + B, t + 2B, ... of
 // the range it loaded.
