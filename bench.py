@@ -135,20 +135,22 @@ def timed_steps(ws, steps, warmup, paths=False):
 
 def persistent_control(dev, dg, batch, tune, target_bits, steps):
     """The same batch and tuning under one persistent cooperative launch per
-    step (grid barrier between rounds) instead of the tuned control: wall
-    and relaxation-kernel ms per step, and whether every target distance
-    has the bits of the tuned run."""
+    step (grid barrier between rounds) instead of the tuned control, with
+    the entry's persistent overrides (its wave bound): wall and
+    relaxation-kernel ms per step, and whether every target distance has
+    the bits of the tuned run."""
+    t = dict(tune, **{k: v for k, v in (tune.get("persistent") or {}).items() if k == "min_waves"})
     ws = bfn.Workspace(dev)
     try:
-        ws.submit(dg, batch, bfn.run_options(engine=tune["engine"], control_mode=bfn.PERSISTENT,
-                                             block_size=tune["block"]),
-                  bfn.tuning(tile_sweeps=tune["sweeps"], items_per_claim=tune["items_per_claim"],
-                             boundary_mode=tune["boundary_mode"], min_waves=tune["min_waves"],
-                             push_window=tune.get("push_window", 0.0)))
+        ws.submit(dg, batch, bfn.run_options(engine=t["engine"], control_mode=bfn.PERSISTENT,
+                                             block_size=t["block"]),
+                  bfn.tuning(tile_sweeps=t["sweeps"], items_per_claim=t["items_per_claim"],
+                             boundary_mode=t["boundary_mode"], min_waves=t["min_waves"],
+                             push_window=t.get("push_window", 0.0)))
         st, _, _ = ws.status()
         bits = np.array([p["distance"] for p in ws.paths()], np.float32).view(np.uint32)
         elapsed, relax_ms, _, _ = timed_steps(ws, steps, 1)
-        return {"control": "persistent_cooperative", "steps": steps,
+        return {"control": "persistent_cooperative", "min_waves": t["min_waves"], "steps": steps,
                 "ms_per_step": elapsed * 1e3 / steps, "kernel_ms_per_step": float(np.mean(relax_ms)),
                 "rounds": int(ws.counters().rounds_executed), "converged": bool(st.converged),
                 "same_target_bits": bool(np.array_equal(bits, target_bits))}
