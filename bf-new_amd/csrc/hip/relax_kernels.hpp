@@ -274,11 +274,9 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
 template <int B, int EPT, bool I>
 __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b, const SlotDesc& d,
                            std::uint32_t r, bool frontier, const Lds& l,
-                           RecordRegs<B, EPT>& internal, std::uint32_t& regs_key,
                            VisitStats<I>& st, std::uint32_t& queued_min) {
   const std::uint32_t tid = threadIdx.x;
   st.start();
-  const std::uint32_t T = d.tile;
   const std::uint32_t q = d.query;
   const std::uint32_t v0 = d.v0;
   const std::uint32_t npt = d.npt;
@@ -292,12 +290,14 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
   const std::uint32_t e1 = d.e1;
   const std::uint32_t ei = d.ei;
   const bool resident = ei - e0 <= static_cast<std::uint32_t>(B * EPT);
-  // The internal records stay in registers from a previous visit of T;
-  // otherwise they are requested first, so that they travel together with
-  // the label loads below (one round trip for both).
-  const std::uint32_t key = 2 * T + 1;
-  if (resident && regs_key != key) load_records<B, EPT>(internal, g.csr_rec, e0, ei);
-  regs_key = resident ? key : kNone;
+  // The internal records are requested first, so that they travel together
+  // with the label loads below (one round trip for both).  They live only
+  // through the tile closure: kept in registers across visits of the same
+  // tile they would stay live through the boundary phase too, and those 16
+  // VGPRs cost the 256-thread push-only kernel two of its eight workgroups
+  // per CU.
+  RecordRegs<B, EPT> internal;
+  if (resident) load_records<B, EPT>(internal, g.csr_rec, e0, ei);
 
   const bool proposals = b.boundary_mode == 0;
   for (std::uint32_t k = tid; k < KO; k += B) {
@@ -535,8 +535,9 @@ template <int B, int EPT, bool I, int M>
 __device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch& b, std::uint32_t r,
                               std::uint32_t n, std::uint32_t limit, const Lds& l) {
   const std::uint32_t chunk = b.items_per_claim ? b.items_per_claim : 1u;
-  RecordRegs<B, EPT> regs;
-  std::uint32_t regs_key = kNone;  // tile and orientation of the records in regs
+  // The pull visit's internal records, kept across visits of the same tile.
+  [[maybe_unused]] RecordRegs<B, EPT> regs;
+  [[maybe_unused]] std::uint32_t regs_key = kNone;  // tile of the records in regs
   VisitStats<I> st;
   std::uint32_t queued_min = kInfBits;
   if (threadIdx.x == 0) l.flags[4] = kInfBits;
@@ -565,8 +566,7 @@ __device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch&
         run = l.flags[3] == 0u;
       }
       if (run)
-        visit_push<B, EPT, I>(g, b, d, r, d.engine == kFrontierPush, l, regs, regs_key, st,
-                              queued_min);
+        visit_push<B, EPT, I>(g, b, d, r, d.engine == kFrontierPush, l, st, queued_min);
     }
     if (run) ++st.items;
     __syncthreads();
