@@ -146,11 +146,9 @@ __device__ __forceinline__ std::uint32_t pk_dst(std::uint32_t p) { return p & kP
 // ---------------------------------------------------------------------------
 template <int B, int EPT, bool I>
 __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b, const SlotDesc& d,
-                           std::uint32_t r, const Lds& l, RecordRegs<B, EPT>& internal,
-                           std::uint32_t& regs_key, VisitStats<I>& st) {
+                           std::uint32_t r, const Lds& l, VisitStats<I>& st) {
   const std::uint32_t tid = threadIdx.x;
   st.start();
-  const std::uint32_t T = d.tile;
   const std::uint32_t q = d.query;
   const std::uint32_t npt = d.npt;
   const std::uint32_t nb0 = d.nbr0;
@@ -196,22 +194,15 @@ __device__ __forceinline__ void visit_pull(const DevGraph& g, const DevBatch& b,
     const std::uint32_t cb = as_bits(as_float(l.work[l.slots[pk_k(p)] + pk_src(p)]) + w);
     if (cb < l.own[pk_dst(p)]) atomicMin(&l.own[pk_dst(p)], cb);
   };
-  // The internal records stay in registers from a previous visit of T.
-  const std::uint32_t key = 2 * T;
-  if (resident && regs_key == key) {
+  // The internal records: loaded once here, and kept in registers through
+  // the tile closure when they fit (resident); not kept across visits, so
+  // they hold no registers outside this visit.
+  RecordRegs<B, EPT> internal;
+  for (std::uint32_t base = e0; base < ei; base += B * EPT) {
+    load_records<B, EPT>(internal, g.csc_rec, base, ei);
 #pragma unroll
     for (int j = 0; j < EPT; ++j)
       if (internal.pack[j] != kNone) relax_staged(internal.pack[j], internal.w[j]);
-  } else {
-    for (std::uint32_t base = e0; base < ei; base += B * EPT) {
-      load_records<B, EPT>(internal, g.csc_rec, base, ei);
-#pragma unroll
-      for (int j = 0; j < EPT; ++j)
-        if (internal.pack[j] != kNone) relax_staged(internal.pack[j], internal.w[j]);
-    }
-    // A tile without internal edges loads nothing: the registers still hold
-    // the records regs_key names, which must not be taken for T's.
-    if (ei > e0) regs_key = resident ? key : kNone;
   }
   st.mark(1);
   for (std::uint32_t base = ei; base < e1; base += B * stream_records(B)) {
@@ -535,9 +526,6 @@ template <int B, int EPT, bool I, int M>
 __device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch& b, std::uint32_t r,
                               std::uint32_t n, std::uint32_t limit, const Lds& l) {
   const std::uint32_t chunk = b.items_per_claim ? b.items_per_claim : 1u;
-  // The pull visit's internal records, kept across visits of the same tile.
-  [[maybe_unused]] RecordRegs<B, EPT> regs;
-  [[maybe_unused]] std::uint32_t regs_key = kNone;  // tile of the records in regs
   VisitStats<I> st;
   std::uint32_t queued_min = kInfBits;
   if (threadIdx.x == 0) l.flags[4] = kInfBits;
@@ -549,7 +537,7 @@ __device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch&
     // engines, so it does not happen).  Nothing is done here: any extra
     // global operation on this path costs the variant 18 VGPRs.
     if (d.engine == kJacobiPull) {
-      if constexpr (M != kEnginesPush) visit_pull<B, EPT, I>(g, b, d, r, l, regs, regs_key, st);
+      if constexpr (M != kEnginesPush) visit_pull<B, EPT, I>(g, b, d, r, l, st);
     } else if constexpr (M != kEnginesPull) {
       if (b.prio) {
         if (threadIdx.x == 0) {
