@@ -567,8 +567,8 @@ __device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch&
     // dispatcher places workgroups on the XCDs round-robin), then from the
     // others in turn.  Consecutive slots share a tile, so the slots of one
     // tile are visited by one XCD, close together in time, and a chunk of
-    // them by one workgroup (internal records stay in registers).  Thread 0
-    // keeps the next claim in flight while a chunk is visited.
+    // them by one workgroup (the tile's records stay in that XCD's L2).
+    // Thread 0 keeps the next claim in flight while a chunk is visited.
     DevControl* c = b.control;
     const std::uint32_t home = blockIdx.x % kBarrierGroups;
     std::uint32_t tried = 0;  // thread 0: segments exhausted so far
@@ -732,8 +732,8 @@ hipError_t occupancy_relax(bool persistent, std::size_t lds_bytes, int* blocks_p
 }
 
 // Instantiated kernel variants.  Block sizes come with the registers per
-// thread that hold a tile's internal records (B * EPT = 2048: 512-vertex
-// tiles with four internal out-edges per vertex stay register-resident).
+// thread that hold a tile's internal records through its closure (B * EPT =
+// 2048: 512-vertex tiles with four internal out-edges per vertex fit).
 // The 256- and 512-thread blocks also come bounded to at least 6, 7 or 8 waves
 // per SIMD (fewer registers per lane, more visits in flight); instrumented
 // runs use the unbounded variant.
