@@ -405,7 +405,9 @@ int32_t bfn_graph_upload(bfn_device device, const bfn_graph_layout32* L, bfn_gra
   if (e == hipSuccess) e = g->csc_pack.reserve(sizeof(std::uint32_t) * e_words);
   if (e == hipSuccess) e = g->csr_pack.reserve(sizeof(std::uint32_t) * e_words);
   if (e == hipSuccess) e = g->csc_rec.reserve(sizeof(uint2) * e_words);
-  if (e == hipSuccess) e = g->csr_rec.reserve(sizeof(uint2) * e_words);
+  // The push visits read the CSR stream two records per 16-byte load from
+  // even positions, so it holds one record to spare.
+  if (e == hipSuccess) e = g->csr_rec.reserve(sizeof(uint2) * (e_words + 1));
   if (e == hipSuccess) e = g->csc_perm.reserve(sizeof(std::uint32_t) * e_words);
   if (e == hipSuccess) e = g->csr_perm.reserve(sizeof(std::uint32_t) * e_words);
   if (e == hipSuccess) e = g->csc_intra.reserve(sizeof(std::uint32_t) * (T + 1));

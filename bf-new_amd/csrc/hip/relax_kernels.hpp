@@ -79,6 +79,7 @@ __device__ __forceinline__ Lds carve(std::uint32_t* base, std::uint32_t max_v,
 // chunk, and at six the 256-thread push-only kernel fits 79 VGPRs: six
 // workgroups per CU, where eight records per chunk needed 85 (five).
 constexpr int kPushChunk = 6;
+static_assert(kPushChunk % 2 == 0, "boundary records are loaded in pairs");
 
 // Edge records in registers: thread t holds records t, t + B, t + 2B, ... of
 // the range it loaded.
@@ -434,18 +435,21 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
       const std::uint32_t rb = l.bnd[i];
       const std::uint32_t re = i + 1 < npt ? l.bnd[i + 1] : e1;
       loaded += re - rb;
-      for (std::uint32_t r0 = rb; r0 < re; r0 += kPushChunk) {
+      // Chunks start at even positions, so a lane's records arrive two per
+      // 16-byte load (the stream holds one record to spare past its end).
+      for (std::uint32_t r0 = rb & ~1u; r0 < re; r0 += kPushChunk) {
         std::uint32_t pk[kPushChunk];
         float w[kPushChunk];
 #pragma unroll
-        for (int c = 0; c < kPushChunk; ++c) {
-          pk[c] = kNone;
-          w[c] = 0.0f;
-          if (r0 + c < re) {
-            const uint2 x = g.csr_rec[r0 + c];
-            pk[c] = x.x;
-            w[c] = as_float(x.y);
-          }
+        for (int c = 0; c < kPushChunk; c += 2) {
+          uint4 x = make_uint4(kNone, 0u, kNone, 0u);
+          if (r0 + c < re) x = *reinterpret_cast<const uint4*>(g.csr_rec + r0 + c);
+          const bool in0 = r0 + c >= rb && r0 + c < re;
+          const bool in1 = r0 + c + 1 >= rb && r0 + c + 1 < re;
+          pk[c] = in0 ? x.x : kNone;
+          w[c] = in0 ? as_float(x.y) : 0.0f;
+          pk[c + 1] = in1 ? x.z : kNone;
+          w[c + 1] = in1 ? as_float(x.w) : 0.0f;
         }
         auto target = [&](std::uint32_t p) -> std::uint32_t* {
           const unsigned long long off = (static_cast<unsigned long long>(l.offs[2 * pk_k(p) + 1]) << 32) |

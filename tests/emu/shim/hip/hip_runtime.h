@@ -44,6 +44,10 @@ struct dim3 {
 };
 struct uint2 { std::uint32_t x, y; };
 inline uint2 make_uint2(std::uint32_t a, std::uint32_t b) { return uint2{a, b}; }
+struct alignas(16) uint4 { std::uint32_t x, y, z, w; };
+inline uint4 make_uint4(std::uint32_t a, std::uint32_t b, std::uint32_t c, std::uint32_t d) {
+  return uint4{a, b, c, d};
+}
 
 struct emu_graph;
 struct emu_event;
