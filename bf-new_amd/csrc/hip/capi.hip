@@ -1132,6 +1132,10 @@ int32_t bfn_batch_submit(bfn_workspace ws, bfn_graph graph, const bfn_batch_desc
     blocks = static_cast<std::uint32_t>(placed) * cus;
     if (o->grid_policy == 1) blocks = std::min(blocks, o->blocks_per_wgp * cus);
   }
+  // A round holds each slot at most once, so workgroups beyond the slot
+  // count never find work; a persistent grid of them would still wait at
+  // every barrier (small batches: one query per launch).
+  blocks = std::min(blocks, std::max(S, 1u));
   ws->block = o->block_size;
   ws->max_v = max_v;
   ws->work_words = work_words;

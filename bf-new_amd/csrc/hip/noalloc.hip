@@ -296,6 +296,7 @@ int32_t bfn_batch_run_static(bfn_graph graph, const bfn_batch_desc* d, const bfn
     grid = static_cast<std::uint32_t>(placed) * gh.cu_count;
     if (o->grid_policy == 1) grid = std::min(grid, o->blocks_per_wgp * gh.cu_count);
   }
+  grid = std::min(grid, std::max(S, 1u));  // no more workgroups than slots
   auto h2d = [&](std::size_t off, const void* src, std::size_t bytes) {
     return bytes ? hipMemcpyAsync(m + off, src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
   };
