@@ -333,8 +333,14 @@ hipError_t bfn_server_s::step(std::uint32_t admit_count, std::uint32_t tile_coun
                        d_upload_tiles, d_onpath, d_frame_base, d_active, slab_words, kin, kout);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
+  // No more workgroups than live slots (a round visits each at most once);
+  // `active` belongs to the worker thread that runs this step.
+  std::uint64_t live = 0;
+  for (const Active& a : active) live += a.slot_count;
+  const std::uint32_t chunk_grid =
+      static_cast<std::uint32_t>(std::min<std::uint64_t>(grid, std::max<std::uint64_t>(live, 1)));
   for (std::uint32_t r = 0; r < options.rounds_per_chunk; ++r) {
-    if ((e = relax_launch(block, false, false, grid, lds_bytes, max_v, work_words, gview, view,
+    if ((e = relax_launch(block, false, false, chunk_grid, lds_bytes, max_v, work_words, gview, view,
                           s)) != hipSuccess)
       return e;
     launches += 2;
