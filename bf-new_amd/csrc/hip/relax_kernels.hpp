@@ -611,7 +611,12 @@ __device__ __forceinline__ void process_round(const DevGraph& g, const DevBatch&
     }
   } else {
     const std::uint32_t li = r % 3;
-    for (;;) {
+    // The first ceil(n / chunk) workgroups claim until the list is exhausted;
+    // the others would only find it exhausted, and skip the claim counter
+    // (in a small round most of the grid would otherwise queue atomics on
+    // its one line).
+    const bool claims = static_cast<unsigned long long>(blockIdx.x) * chunk < n;
+    for (; claims;) {
       if (threadIdx.x == 0) l.flags[0] = atomic_add_agent(&b.control->claim[li], chunk);
       __syncthreads();
       const std::uint32_t start = l.flags[0];
