@@ -306,13 +306,34 @@ __device__ __forceinline__ void visit_push(const DevGraph& g, const DevBatch& b,
       l.offs[2 * k + 1] = static_cast<std::uint32_t>(off >> 32);
     }
   }
-  for (std::uint32_t i = tid; i < npt; i += B) {
-    const std::uint32_t x = load_agent(column + i);
-    l.own[i] = x;
-    l.orig[i] = x;
-    // Dense pushes every finite vertex: an infinite snapshot admits them all.
-    l.work[i] = frontier ? snapshot[i] : kInfBits;
-    if (!proposals) l.bnd[i] = g.csr_bnd[v0 + i];
+  // Dense pushes every finite vertex: an infinite snapshot admits them all.
+  if (((npt | v0) & 1u) == 0) {
+    // Even tile size and first vertex: labels, snapshot and record starts
+    // two per 8-byte load (pool offsets are sums of 2 npt, so all even).
+    for (std::uint32_t i = 2 * tid; i < npt; i += 2 * B) {
+      const unsigned long long x = __hip_atomic_load(
+          reinterpret_cast<const unsigned long long*>(column + i), __ATOMIC_RELAXED,
+          __HIP_MEMORY_SCOPE_AGENT);
+      l.own[i] = l.orig[i] = static_cast<std::uint32_t>(x);
+      l.own[i + 1] = l.orig[i + 1] = static_cast<std::uint32_t>(x >> 32);
+      const uint2 sn = frontier ? *reinterpret_cast<const uint2*>(snapshot + i)
+                                : make_uint2(kInfBits, kInfBits);
+      l.work[i] = sn.x;
+      l.work[i + 1] = sn.y;
+      if (!proposals) {
+        const uint2 bd = *reinterpret_cast<const uint2*>(g.csr_bnd + v0 + i);
+        l.bnd[i] = bd.x;
+        l.bnd[i + 1] = bd.y;
+      }
+    }
+  } else {
+    for (std::uint32_t i = tid; i < npt; i += B) {
+      const std::uint32_t x = load_agent(column + i);
+      l.own[i] = x;
+      l.orig[i] = x;
+      l.work[i] = frontier ? snapshot[i] : kInfBits;
+      if (!proposals) l.bnd[i] = g.csr_bnd[v0 + i];
+    }
   }
   if (tid == 0) l.flags[1] = 0;
   if (proposals) {
